@@ -1,0 +1,193 @@
+#!/usr/bin/env python
+"""Throughput benchmark of the LIC hot path on one MI355X.
+
+Workload (default): plain LIC at scale — 4096x4096 float32, texture U[0,1),
+u and v sums of 8 random Fourier modes, Gaussian kernel of 129 taps
+(sigma 32), boundaries x periodic / y closed, velocity mode, 5 iterations.
+One step = one full ``convolve`` call (pack, trace, replays) on inputs that
+are already resident in HBM as device tensors.
+
+Prints ONE JSON line:
+  value     Gpixel-steps/s = H*W*(L-1)*iterations per step / step time
+  roofline  for the kernel with the most total time in the timed region:
+            algorithmic bytes per launch (the reference's gathers per pass:
+            (3*(L-1)+2)*sizeof(T) per pixel) / average launch time measured
+            with HIP events on the launch stream, against 8 TB/s HBM;
+            `traffic` = measured HBM bytes per launch from the committed
+            rocprofv3 counter summary (null when absent)
+  cpu_baseline  the CPU oracle (a restatement of the reference's
+            single-threaded algorithm) on a bounded band of rows of pass 1.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles(kernel: str, workload: str) -> float | None:
+    """HBM bytes per launch from the committed counter summary, if any."""
+    path = ROOT / "profiles" / f"traffic_{kernel}_{workload}.json"
+    if not path.is_file():
+        return None
+    try:
+        data = json.loads(path.read_text())
+        return float(data["hbm_bytes_per_launch"])
+    except (ValueError, KeyError):
+        return None
+
+
+def cpu_baseline(wl, rows: int) -> dict:
+    """Time the CPU oracle on pass 1 of a band of `rows` rows (one thread)."""
+    from oracle import cpu
+
+    ny, nx = wl.shape
+    r0 = max(0, ny // 2 - rows // 2)
+    r1 = min(ny, r0 + rows)
+    t0 = time.perf_counter()
+    cpu.convolve(wl.texture, wl.u, wl.v, kernel=wl.kernel, uv_mode=wl.uv_mode,
+                 boundaries=wl.boundaries, iterations=1, rows=(r0, r1), threads=1)
+    dt = time.perf_counter() - t0
+    steps = (r1 - r0) * nx * (wl.kernel.size - 1)
+    return {
+        "value": steps / dt / 1e9,
+        "unit": "Gpixel-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"pass 1 of rows [{r0},{r1}) x {nx} cols "
+                  f"({steps / 1e6:.1f} M pixel-steps, {dt:.2f} s), single thread, "
+                  f"host CPU: {os.cpu_count()} logical cores visible",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="plain4096")
+    ap.add_argument("--cpu-rows", type=int, default=2048,
+                    help="rows of pass 1 timed on the CPU oracle (0 = skip)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the LIC path has "
+              "no multi-GPU implementation in this build; see DESIGN.md for how it would "
+              "shard.", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import rlic_amd
+    from rlic_amd import _core, _native
+    from rlic_amd.workloads import WORKLOADS
+
+    assert torch.cuda.is_available(), "no ROCm device visible"
+    dev = torch.device("cuda:0")
+    log(f"bench: building workload {args.workload}")
+    wl = WORKLOADS[args.workload]()
+    fn = _core.convolve_f32 if wl.texture.dtype == np.float32 else _core.convolve_f64
+    t = [torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in
+         (wl.texture, wl.u, wl.v, wl.kernel)]
+    bs = rlic_amd._boundaries.BoundarySet.from_spec(wl.boundaries)
+    bpair = (bs.x, bs.y)
+
+    def step():
+        return fn(t[0], (t[1], t[2], wl.uv_mode), t[3], bpair, wl.iterations)
+
+    log(f"bench: {args.warmup} warmup steps")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    _native.profiler_enable(True)
+    _native.profiler_read()
+    log(f"bench: {args.steps} timed steps")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = _native.profiler_read()
+    _native.profiler_enable(False)
+    log(f"bench: kernel times {json.dumps(prof)}")
+
+    ms_per_step = elapsed / args.steps * 1e3
+    value = wl.pixel_steps / (ms_per_step / 1e3) / 1e9
+
+    dominant = max(prof, key=lambda k: prof[k]["total_ms"]) if prof else None
+    roofline = None
+    if dominant:
+        avg_s = prof[dominant]["total_ms"] / prof[dominant]["count"] / 1e3
+        b_alg = wl.algorithmic_bytes_per_pass()
+        achieved = b_alg / avg_s / 1e9
+        roofline = {
+            "bound": "hbm",
+            "kernel": dominant,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profiles(dominant, args.workload),
+            "algorithmic_bytes_per_launch": b_alg,
+            "avg_launch_ms": round(avg_s * 1e3, 4),
+            "kernel_ms_per_step": {k: round(v["total_ms"] / args.steps, 4)
+                                   for k, v in prof.items()},
+        }
+
+    cpu = None
+    if args.cpu_rows > 0:
+        log(f"bench: CPU oracle baseline on {args.cpu_rows} rows")
+        cpu = cpu_baseline(wl, args.cpu_rows)
+
+    ny, nx = wl.shape
+    line = {
+        "metric": "Gpixel-steps/s",
+        "value": round(value, 3),
+        "unit": "Gpixel-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if wl.texture.dtype == np.float32 else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": wl.name,
+            "shape": [ny, nx],
+            "kernel_len": int(wl.kernel.size),
+            "iterations": wl.iterations,
+            "boundaries": wl.boundaries,
+            "uv_mode": wl.uv_mode,
+            "pixel_steps_per_step": wl.pixel_steps,
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

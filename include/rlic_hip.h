@@ -1,0 +1,123 @@
+/*
+ * rlic_hip.h — C ABI of the MI355X (gfx950) line integral convolution library
+ * librlic_hip.so.
+ *
+ * The entry points replace the reference's PyO3 functions
+ *   rlic._core.convolve_f32 / convolve_f64      (src/lib.rs:450-482,
+ *                                                src/rlic/_core.pyi:8-29)
+ * which the Python API calls at exactly one site (src/rlic/_lib.py:235).
+ * Arguments map one to one:
+ *   texture                    -> texture + element strides (sy, sx)
+ *   uv = (u, v, uv_mode)       -> u, v + strides, uv_mode (RLIC_UV_*)
+ *   kernel                     -> kernel, kernel_len
+ *   boundaries ((xl,xr),(yl,yr)) -> x_left, x_right, y_left, y_right (RLIC_BOUND_*)
+ *   iterations                 -> iterations
+ *   return value (new ndarray) -> caller-allocated C-order `out` (ny*nx)
+ *
+ * Conventions
+ *   - Plain pointers and sizes only.  Strides are in ELEMENTS and may be
+ *     negative or zero (broadcast views); element (i, j) of an array is at
+ *     base[i*sy + j*sx].  `out` is always C-contiguous and must not overlap
+ *     any input.
+ *   - `location` says where ALL array pointers live: RLIC_HOST (pageable or
+ *     pinned host memory; the call uploads, computes, downloads and returns
+ *     when `out` is filled) or RLIC_DEVICE (device memory of the current HIP
+ *     device; work is enqueued on `stream` and the call returns without
+ *     waiting, like any stream-ordered HIP operation).
+ *   - `stream` is a hipStream_t (NULL = the legacy default stream).
+ *   - Return 0 on success or a negative RLIC_ERR_* code; nothing aborts.
+ *     rlic_last_error() gives a message for the calling thread's last failure.
+ *   - Thread-safe.  The library keeps no pointer past return.
+ *
+ * profile (bit flags) selects the reference's compile-time arithmetic
+ * profile (Cargo features, Cargo.toml:27-30):
+ *   RLIC_PROFILE_FMA        `fma`        mul_add at src/lib.rs:177,231,359,382
+ *   RLIC_PROFILE_BRANCHLESS `branchless` time_to_next_pixel at src/lib.rs:168-179
+ *   RLIC_PROFILE_DEFAULT = both, the Cargo default (Cargo.toml:28).
+ */
+#ifndef RLIC_HIP_H
+#define RLIC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  RLIC_UV_VELOCITY = 0,      /* src/lib.rs:16  */
+  RLIC_UV_POLARIZATION = 1   /* src/lib.rs:17  */
+};
+
+enum {
+  RLIC_BOUND_CLOSED = 0,     /* src/lib.rs:57 */
+  RLIC_BOUND_PERIODIC = 1    /* src/lib.rs:58 */
+};
+
+enum {
+  RLIC_HOST = 0,
+  RLIC_DEVICE = 1
+};
+
+enum {
+  RLIC_PROFILE_FMA = 1,
+  RLIC_PROFILE_BRANCHLESS = 2,
+  RLIC_PROFILE_DEFAULT = 3
+};
+
+enum {
+  RLIC_OK = 0,
+  RLIC_ERR_INVALID_ARGUMENT = -1,
+  RLIC_ERR_OUT_OF_MEMORY = -2,
+  RLIC_ERR_HIP = -3,
+  RLIC_ERR_EMPTY_KERNEL = -4  /* the reference aborts the process here
+                                 (kernel[[kmid]] out of bounds, src/lib.rs:382) */
+};
+
+/* Replaces rlic._core.convolve_f32 (src/lib.rs:450-465). */
+int rlic_convolve_f32(const float* texture, int64_t tex_sy, int64_t tex_sx,
+                      const float* u, int64_t u_sy, int64_t u_sx,
+                      const float* v, int64_t v_sy, int64_t v_sx,
+                      int64_t ny, int64_t nx,
+                      const float* kernel, int64_t kernel_len,
+                      int uv_mode,
+                      int x_left, int x_right, int y_left, int y_right,
+                      int64_t iterations,
+                      float* out,
+                      int location, int profile, void* stream);
+
+/* Replaces rlic._core.convolve_f64 (src/lib.rs:467-482). */
+int rlic_convolve_f64(const double* texture, int64_t tex_sy, int64_t tex_sx,
+                      const double* u, int64_t u_sy, int64_t u_sx,
+                      const double* v, int64_t v_sy, int64_t v_sx,
+                      int64_t ny, int64_t nx,
+                      const double* kernel, int64_t kernel_len,
+                      int uv_mode,
+                      int x_left, int x_right, int y_left, int y_right,
+                      int64_t iterations,
+                      double* out,
+                      int location, int profile, void* stream);
+
+/* Message for the calling thread's most recent failed call ("" if none).
+ * No counterpart in the reference, whose errors abort (Cargo.toml:23). */
+const char* rlic_last_error(void);
+
+/* Kernel timing.  No counterpart in the reference (which has no timers).
+ * rlic_profiler_enable(1): every later kernel launch of the library is
+ * bracketed by two hipEvents recorded on the stream it runs on.
+ * rlic_profiler_read: resolves the recorded events (waits for them),
+ * writes a JSON object {"<kernel>": {"count": n, "total_ms": t}, ...} into
+ * buf (NUL-terminated) and clears the record when len is large enough,
+ * keeps the record otherwise (e.g. buf == NULL to size the buffer), and
+ * returns the number of bytes needed including the NUL. */
+int rlic_profiler_enable(int on);
+int64_t rlic_profiler_read(char* buf, int64_t len);
+
+/* Library version string.  No counterpart in the reference. */
+const char* rlic_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RLIC_HIP_H */

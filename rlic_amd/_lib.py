@@ -1,0 +1,209 @@
+"""Public API: ``convolve`` (drop-in for ``rlic.convolve``).
+
+Signature, defaults, validation order, error types and messages follow the
+reference's ``src/rlic/_lib.py:37-235`` (pinned by its
+``tests/test_exceptions.py``):
+
+* every validator runs before anything returns (``_lib.py:135-200``);
+* one problem is raised bare, several as an ExceptionGroup
+  "Invalid inputs were received." (``_lib.py:202-205``);
+* ``iterations == 0`` returns a copy of ``texture`` (``_lib.py:208-209``);
+* otherwise the call dispatches on dtype to the native
+  ``convolve_f32`` / ``convolve_f64`` (``_lib.py:228-235``).
+
+Beyond the reference, the arrays may also be PyTorch tensors on a ROCm
+device: the same checks then run on the device and the result is a device
+tensor.
+"""
+
+from __future__ import annotations
+
+__all__ = ["convolve"]
+
+import sys
+from typing import TYPE_CHECKING, Any
+
+import numpy as np
+
+from rlic_amd._boundaries import BoundarySet
+from rlic_amd._core import convolve_f32, convolve_f64
+
+if sys.version_info >= (3, 11):
+    _ExceptionGroup = ExceptionGroup  # noqa: F821
+else:  # pragma: no cover - exercised on the 3.10 interpreter
+    from exceptiongroup import ExceptionGroup as _ExceptionGroup
+
+if TYPE_CHECKING:
+    from rlic_amd._boundaries import BoundarySpec
+    from rlic_amd._typing import UVMode
+
+_KNOWN_UV_MODES = ["velocity", "polarization"]
+_SUPPORTED_DTYPES: list[np.dtype] = [np.dtype("float32"), np.dtype("float64")]
+
+
+def _is_torch(x: Any) -> bool:
+    return type(x).__module__.split(".", 1)[0] == "torch"
+
+
+def _np_dtype(arr: Any) -> Any:
+    """The array's dtype as a NumPy dtype (torch dtypes mapped when possible)."""
+    if _is_torch(arr):
+        import torch
+
+        try:
+            return np.dtype(torch.empty((), dtype=arr.dtype).numpy().dtype)
+        except TypeError:  # e.g. bfloat16: no NumPy equivalent
+            return arr.dtype
+    return arr.dtype
+
+
+def _shape(arr: Any) -> tuple[int, ...]:
+    return tuple(int(s) for s in arr.shape)
+
+
+def _any_negative(arr: Any) -> bool:
+    if _is_torch(arr):
+        if arr.is_complex():
+            return False
+        return bool((arr < 0).any())
+    return bool(np.any(arr < 0))
+
+
+def _any_non_finite(arr: Any) -> bool:
+    if _is_torch(arr):
+        import torch
+
+        return bool((~torch.isfinite(arr)).any())
+    return bool(np.any(~np.isfinite(arr)))
+
+
+def _copy(arr: Any) -> Any:
+    return arr.clone() if _is_torch(arr) else arr.copy()
+
+
+def convolve(
+    texture: Any,
+    /,
+    u: Any,
+    v: Any,
+    *,
+    kernel: Any,
+    uv_mode: UVMode = "velocity",
+    boundaries: BoundarySpec = "closed",
+    iterations: int = 1,
+) -> Any:
+    """2-dimensional line integral convolution.
+
+    Apply Line Integral Convolution to a texture array, against a 2D flow
+    (u, v) and via a 1D kernel, on an AMD Instinct GPU.
+
+    Arguments
+    ---------
+    texture: 2D array, positional-only
+      The tracer image to smear along the flow (random noise in general).
+      Elements must not be negative.
+
+    u, v: 2D arrays
+      Horizontal (along axis 1, x) and vertical (along axis 0, y) components
+      of the vector field, same shape as ``texture``.
+
+    kernel: 1D array, keyword-only
+      Weights along a field line.  ``kernel[len(kernel) // 2]`` weighs the
+      starting pixel, later elements the downstream ("future") part of the
+      line and earlier elements the upstream ("past") part.  Values must be
+      finite; negative values are allowed.
+
+    uv_mode: 'velocity' (default) or 'polarization', keyword-only
+      In polarization mode the sign of (u, v) is ignored: along a line each
+      vector is flipped when it points against the previous one.
+
+    boundaries: 'closed' (default), 'periodic', or a dict with keys 'x' and
+      'y' whose values are one of these strings or (left, right) pairs of
+      them.  A 'periodic' side must face a 'periodic' side.
+
+    iterations: int >= 0 (default: 1), keyword-only
+      Number of passes; each pass convolves the previous pass's output.
+
+    Returns
+    -------
+    A new 2D array of the inputs' dtype (float32 or float64).  NumPy inputs
+    give a NumPy array; PyTorch tensors on a ROCm device give a tensor on
+    that device.  ``iterations=0`` returns a copy of ``texture``.
+
+    Raises
+    ------
+    TypeError
+      Unsupported or mismatched dtypes, or an invalid boundary structure.
+    ValueError
+      Invalid values (see the messages).
+    ExceptionGroup
+      When more than one problem is found; all are reported.
+    """
+    problems: list[Exception] = []
+    if iterations < 0:
+        problems.append(
+            ValueError(
+                f"Invalid number of iterations: {iterations}\n"
+                "Expected a strictly positive integer."
+            )
+        )
+    if uv_mode not in _KNOWN_UV_MODES:
+        problems.append(
+            ValueError(f"Invalid uv_mode {uv_mode!r}. Expected one of {_KNOWN_UV_MODES}")
+        )
+
+    dtypes = [_np_dtype(a) for a in (texture, u, v, kernel)]
+    expectation = (
+        "Expected texture, u, v and kernel with identical dtype, from "
+        f"{_SUPPORTED_DTYPES}. Got texture.dtype={dtypes[0]!r}, u.dtype={dtypes[1]!r}, "
+        f"v.dtype={dtypes[2]!r}, kernel.dtype={dtypes[3]!r}"
+    )
+    distinct = list(dict.fromkeys(dtypes))
+    unsupported = [dt for dt in distinct if dt not in _SUPPORTED_DTYPES]
+    if unsupported:
+        problems.append(
+            TypeError(f"Found unsupported data type(s): {unsupported}. {expectation}")
+        )
+    if len(distinct) != 1:
+        problems.append(TypeError(f"Data types mismatch. {expectation}"))
+
+    if texture.ndim != 2:
+        problems.append(
+            ValueError(f"Expected a texture with exactly two dimensions. Got texture.ndim={texture.ndim}")
+        )
+    if _any_negative(texture):
+        problems.append(
+            ValueError("Found invalid texture element(s). Expected only positive values.")
+        )
+    t_shape, u_shape, v_shape = _shape(texture), _shape(u), _shape(v)
+    if u_shape != t_shape or v_shape != t_shape:
+        problems.append(
+            ValueError(
+                "Shape mismatch: expected texture, u and v with identical shapes. "
+                f"Got texture.shape={t_shape}, u.shape={u_shape}, v.shape={v_shape}"
+            )
+        )
+    if kernel.ndim != 1:
+        problems.append(
+            ValueError(f"Expected a kernel with exactly one dimension. Got kernel.ndim={kernel.ndim}")
+        )
+    if _any_non_finite(kernel):
+        problems.append(ValueError("Found non-finite value(s) in kernel."))
+
+    bs = BoundarySet.from_spec(boundaries)
+    if bs is None:
+        problems.append(TypeError(f"Invalid boundary specification {boundaries}"))
+    else:
+        problems.extend(bs.collect_exceptions())
+
+    if len(problems) == 1:
+        raise problems[0]
+    if problems:
+        raise _ExceptionGroup("Invalid inputs were received.", problems)
+
+    assert bs is not None
+    if iterations == 0:
+        return _copy(texture)
+
+    native = convolve_f32 if dtypes[0] == np.dtype("float32") else convolve_f64
+    return native(texture, (u, v, uv_mode), kernel, (bs.x, bs.y), iterations)

@@ -1,0 +1,142 @@
+"""Loader of the native library ``librlic_hip.so`` (HIP kernels + C ABI).
+
+The library is built in-tree (``rlic_amd/_build.py``) and loaded from this
+package directory.  There is no fallback: if the library is missing or does
+not load, importing the compute entry points fails with an ImportError that
+says how to build it.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+LIB_NAME = "librlic_hip.so"
+LIB_PATH = Path(__file__).resolve().parent / LIB_NAME
+
+# Entry points declared in include/rlic_hip.h, with their ctypes signatures.
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+_vp = ctypes.c_void_p
+
+_CONVOLVE_ARGTYPES = [
+    _vp, _i64, _i64,  # texture, sy, sx
+    _vp, _i64, _i64,  # u
+    _vp, _i64, _i64,  # v
+    _i64, _i64,  # ny, nx
+    _vp, _i64,  # kernel, kernel_len
+    _int,  # uv_mode
+    _int, _int, _int, _int,  # x_left, x_right, y_left, y_right
+    _i64,  # iterations
+    _vp,  # out
+    _int, _int, _vp,  # location, profile, stream
+]
+
+SIGNATURES: dict[str, tuple[list, object]] = {
+    "rlic_convolve_f32": (_CONVOLVE_ARGTYPES, _int),
+    "rlic_convolve_f64": (_CONVOLVE_ARGTYPES, _int),
+    "rlic_last_error": ([], ctypes.c_char_p),
+    "rlic_profiler_enable": ([_int], _int),
+    "rlic_profiler_read": ([ctypes.c_char_p, _i64], _i64),
+    "rlic_version": ([], ctypes.c_char_p),
+}
+
+# Error codes (include/rlic_hip.h)
+RLIC_OK = 0
+RLIC_ERR_INVALID_ARGUMENT = -1
+RLIC_ERR_OUT_OF_MEMORY = -2
+RLIC_ERR_HIP = -3
+RLIC_ERR_EMPTY_KERNEL = -4
+
+RLIC_HOST = 0
+RLIC_DEVICE = 1
+
+_lib: ctypes.CDLL | None = None
+
+
+def _share_torch_hip_runtime() -> None:
+    """Make this library and PyTorch-ROCm use one HIP runtime in the process.
+
+    PyTorch ships its own ``libamdhip64.so`` (soname ``libamdhip64.so.7``) and
+    loads it by file name from its ``lib`` directory.  If the system copy were
+    loaded first (as this library's dependency), PyTorch would then load a
+    second HIP runtime next to it and find no device.  Loading PyTorch's copy
+    first (when PyTorch is installed) lets this library's dependency resolve
+    to it by soname, and PyTorch later finds its own file already loaded.
+    ``RLIC_AMD_SYSTEM_HIP=1`` keeps the system runtime instead.
+    """
+    if os.environ.get("RLIC_AMD_SYSTEM_HIP") == "1":
+        return
+    import importlib.util
+    import sys
+
+    if "torch" in sys.modules:
+        return  # PyTorch is loaded: its runtime is the process's runtime already
+    spec = importlib.util.find_spec("torch")
+    if spec is None or not spec.origin:
+        return
+    candidate = Path(spec.origin).parent / "lib" / "libamdhip64.so"
+    if candidate.is_file():
+        try:
+            ctypes.CDLL(str(candidate), mode=ctypes.RTLD_GLOBAL)
+        except OSError:  # pragma: no cover - broken PyTorch install
+            pass
+
+
+def load() -> ctypes.CDLL:
+    """Load (once) and return the native library, with typed signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = Path(os.environ.get("RLIC_AMD_LIB", LIB_PATH))
+    if not path.is_file():
+        raise ImportError(
+            f"{path} is missing: the HIP extension has not been built. "
+            "Build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "or `python -m rlic_amd._build`."
+        )
+    _share_torch_hip_runtime()
+    try:
+        lib = ctypes.CDLL(str(path))
+    except OSError as exc:  # pragma: no cover - depends on the machine
+        raise ImportError(f"cannot load {path}: {exc}") from exc
+    for name, (argtypes, restype) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = argtypes
+        fn.restype = restype
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    msg = load().rlic_last_error()
+    return msg.decode() if msg else ""
+
+
+def profiler_enable(on: bool = True) -> None:
+    """Record hipEvents around every kernel launch of the library."""
+    load().rlic_profiler_enable(1 if on else 0)
+
+
+def profiler_read() -> dict:
+    """Per-kernel launch count and total milliseconds since the last read."""
+    import json
+
+    lib = load()
+    need = lib.rlic_profiler_read(None, 0)
+    buf = ctypes.create_string_buffer(int(need) + 256)
+    lib.rlic_profiler_read(buf, len(buf))
+    return json.loads(buf.value.decode())
+
+
+def raise_for_status(code: int) -> None:
+    """Map a negative C-ABI status to a Python exception."""
+    if code == RLIC_OK:
+        return
+    msg = last_error()
+    if code == RLIC_ERR_OUT_OF_MEMORY:
+        raise MemoryError(msg)
+    if code in (RLIC_ERR_INVALID_ARGUMENT, RLIC_ERR_EMPTY_KERNEL):
+        raise ValueError(msg)
+    raise RuntimeError(f"HIP failure in librlic_hip: {msg} (status {code})")

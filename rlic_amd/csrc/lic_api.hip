@@ -1,0 +1,376 @@
+// C ABI of librlic_hip.so (declared in include/rlic_hip.h).
+//
+// Iteration driver (src/lib.rs:408-443, convolve_iteratively): the
+// reference copies the texture into a C-order buffer, runs `iterations`
+// passes and swaps input/output between passes.  Here:
+//   1. lic_pack builds the packed (u, v, texture) cell array from the
+//      caller's strided arrays (any strides, including transposed views);
+//   2. lic_trace runs pass 1 and, when more passes follow, records the move
+//      of every streamline step;
+//   3. lic_replay runs passes 2..n from the recorded moves, ping-ponging
+//      two device buffers so that the last pass lands in `out`.
+// When the move record would not fit the memory budget, every pass is
+// traced again instead (texture slot re-packed between passes).
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/rlic_hip.h"
+#include "lic_launch.h"
+
+namespace {
+
+thread_local std::string t_last_error;
+
+int set_error(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  t_last_error = buf;
+  return code;
+}
+
+int hip_error(hipError_t e, const char* what) {
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
+    return set_error(RLIC_ERR_OUT_OF_MEMORY, "%s: out of device memory (%s)", what,
+                     hipGetErrorString(e));
+  }
+  return set_error(RLIC_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define RLIC_HIP_TRY(expr, what)                  \
+  do {                                            \
+    hipError_t e_ = (expr);                       \
+    if (e_ != hipSuccess) return hip_error(e_, what); \
+  } while (0)
+
+// Optional kernel timing: when enabled, every kernel launch of the library
+// is bracketed by two hipEvents recorded on the launch's own stream; the
+// pairs are resolved and aggregated per kernel name on read-out.
+struct TimedLaunch {
+  const char* name;
+  hipEvent_t start, stop;
+};
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+std::vector<TimedLaunch> g_prof_pending;
+std::map<std::string, std::pair<int64_t, double>> g_prof_done;  // name -> (count, total ms)
+
+struct LaunchTimer {
+  const char* name;
+  hipStream_t stream;
+  hipEvent_t start = nullptr, stop = nullptr;
+  bool on = false;
+  LaunchTimer(const char* n, hipStream_t s) : name(n), stream(s) {
+    {
+      std::lock_guard<std::mutex> lock(g_prof_mu);
+      on = g_prof_on;
+    }
+    if (!on) return;
+    if (hipEventCreate(&start) != hipSuccess || hipEventCreate(&stop) != hipSuccess ||
+        hipEventRecord(start, stream) != hipSuccess) {
+      on = false;
+    }
+  }
+  ~LaunchTimer() {
+    if (!on) return;
+    if (hipEventRecord(stop, stream) != hipSuccess) return;
+    std::lock_guard<std::mutex> lock(g_prof_mu);
+    g_prof_pending.push_back({name, start, stop});
+  }
+};
+
+// Stream-ordered device allocations from the device's default memory pool,
+// released on the same stream when the scope ends.  The pool keeps freed
+// memory cached for the next call (release threshold raised once per
+// device).
+void configure_pool(int dev) {
+  static std::mutex mu;
+  static bool done[256] = {};
+  std::lock_guard<std::mutex> lock(mu);
+  if (dev < 0 || dev >= 256 || done[dev]) return;
+  hipMemPool_t pool;
+  if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+    uint64_t threshold = UINT64_MAX;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
+  }
+  done[dev] = true;
+}
+
+struct DevMem {
+  void* ptr = nullptr;
+  hipStream_t stream = nullptr;
+  DevMem() = default;
+  DevMem(const DevMem&) = delete;
+  DevMem& operator=(const DevMem&) = delete;
+  ~DevMem() {
+    if (ptr) (void)hipFreeAsync(ptr, stream);
+  }
+  hipError_t alloc(size_t bytes, hipStream_t s) {
+    stream = s;
+    if (bytes == 0) bytes = 1;
+    return hipMallocAsync(&ptr, bytes, s);
+  }
+};
+
+// Element-offset extent of a strided 2-D view: [lo, hi].
+void view_span(int64_t ny, int64_t nx, int64_t sy, int64_t sx, int64_t& lo, int64_t& hi) {
+  lo = 0;
+  hi = 0;
+  const int64_t dy = (ny - 1) * sy, dx = (nx - 1) * sx;
+  if (dy < 0) lo += dy; else hi += dy;
+  if (dx < 0) lo += dx; else hi += dx;
+}
+
+int env_int(const char* name, int dflt) {
+  const char* s = std::getenv(name);
+  if (!s || !*s) return dflt;
+  return std::atoi(s);
+}
+
+template <typename T>
+int run_on_device(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t usy, int64_t usx,
+                  const T* v, int64_t vsy, int64_t vsx, int32_t ny, int32_t nx,
+                  const T* kernel, int32_t klen, int uv_mode, const rlic::FieldGeom& g,
+                  int64_t iterations, T* out, int profile, hipStream_t stream) {
+  if (iterations == 0) {
+    LaunchTimer timer("lic_copy2d", stream);
+    RLIC_HIP_TRY(rlic::launch_copy2d<T>(tex, tsy, tsx, ny, nx, out, stream), "copy");
+    return RLIC_OK;
+  }
+  const int64_t n = static_cast<int64_t>(ny) * nx;
+  DevMem cells;
+  RLIC_HIP_TRY(cells.alloc(sizeof(rlic::Cell<T>) * n, stream), "allocating packed cells");
+  auto* c = static_cast<rlic::Cell<T>*>(cells.ptr);
+  {
+    LaunchTimer timer("lic_pack", stream);
+    RLIC_HIP_TRY(
+        rlic::launch_pack<T>(tex, tsy, tsx, u, usy, usx, v, vsy, vsx, ny, nx, c, stream),
+        "pack");
+  }
+  const rlic::TileMap m = rlic::make_tile_map(ny, nx, env_int("RLIC_TILE_BAND", 16));
+  if (iterations == 1) {
+    LaunchTimer timer("lic_trace", stream);
+    RLIC_HIP_TRY(rlic::launch_trace<T>(c, kernel, klen, g, m, uv_mode, profile, out, nullptr,
+                                       stream),
+                 "trace");
+    return RLIC_OK;
+  }
+  DevMem tmp_mem;
+  RLIC_HIP_TRY(tmp_mem.alloc(sizeof(T) * n, stream), "allocating iteration buffer");
+  T* tmp = static_cast<T*>(tmp_mem.ptr);
+
+  // Move record: code_words(klen) words per pixel slot.
+  const size_t code_bytes = sizeof(uint32_t) * static_cast<size_t>(rlic::code_words(klen)) *
+                            static_cast<size_t>(m.n_tiles) * rlic::kBlock;
+  bool replay = env_int("RLIC_REPLAY", 1) != 0;
+  if (replay) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+    replay = code_bytes <= free_b / 2;
+  }
+  DevMem code_mem;
+  if (replay && code_mem.alloc(code_bytes, stream) != hipSuccess) {
+    (void)hipGetLastError();
+    code_mem.ptr = nullptr;
+    replay = false;
+  }
+  if (replay) {
+    auto* codes = static_cast<uint32_t*>(code_mem.ptr);
+    // pass `it` (1-based) writes A when odd, B when even; the last lands in out
+    T* a = ((iterations - 1) % 2 == 0) ? out : tmp;
+    T* b = (a == out) ? tmp : out;
+    {
+      LaunchTimer timer("lic_trace", stream);
+      RLIC_HIP_TRY(
+          rlic::launch_trace<T>(c, kernel, klen, g, m, uv_mode, profile, a, codes, stream),
+          "trace");
+    }
+    for (int64_t it = 2; it <= iterations; ++it) {
+      const T* src = (it % 2 == 0) ? a : b;
+      T* dst = (it % 2 == 0) ? b : a;
+      LaunchTimer timer("lic_replay", stream);
+      RLIC_HIP_TRY(rlic::launch_replay<T>(src, kernel, klen, g, m, profile, dst, codes, stream),
+                   "replay");
+    }
+    return RLIC_OK;
+  }
+  for (int64_t it = 1; it <= iterations; ++it) {
+    T* dst = (it == iterations) ? out : tmp;
+    {
+      LaunchTimer timer("lic_trace", stream);
+      RLIC_HIP_TRY(rlic::launch_trace<T>(c, kernel, klen, g, m, uv_mode, profile, dst, nullptr,
+                                         stream),
+                   "trace");
+    }
+    if (it != iterations) {
+      LaunchTimer timer("lic_repack_texture", stream);
+      RLIC_HIP_TRY(rlic::launch_repack_texture<T>(tmp, static_cast<int32_t>(n), c, stream),
+                   "repack");
+    }
+  }
+  return RLIC_OK;
+}
+
+template <typename T>
+int convolve_entry(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t usy, int64_t usx,
+                   const T* v, int64_t vsy, int64_t vsx, int64_t ny, int64_t nx, const T* kernel,
+                   int64_t klen, int uv_mode, int xl, int xr, int yl, int yr, int64_t iterations,
+                   T* out, int location, int profile, void* stream_ptr) {
+  t_last_error.clear();
+  if (ny < 0 || nx < 0) return set_error(RLIC_ERR_INVALID_ARGUMENT, "negative image size");
+  if (klen < 0) return set_error(RLIC_ERR_INVALID_ARGUMENT, "negative kernel length");
+  if (iterations < 0)
+    return set_error(RLIC_ERR_INVALID_ARGUMENT, "Invalid number of iterations: %lld",
+                     static_cast<long long>(iterations));
+  if (uv_mode != RLIC_UV_VELOCITY && uv_mode != RLIC_UV_POLARIZATION)
+    return set_error(RLIC_ERR_INVALID_ARGUMENT, "unknown uv_mode %d", uv_mode);
+  for (int b : {xl, xr, yl, yr}) {
+    if (b != RLIC_BOUND_CLOSED && b != RLIC_BOUND_PERIODIC)
+      return set_error(RLIC_ERR_INVALID_ARGUMENT, "unknown boundary %d", b);
+  }
+  if (location != RLIC_HOST && location != RLIC_DEVICE)
+    return set_error(RLIC_ERR_INVALID_ARGUMENT, "unknown memory location %d", location);
+  if (profile < 0 || profile > 3)
+    return set_error(RLIC_ERR_INVALID_ARGUMENT, "unknown arithmetic profile %d", profile);
+  const int64_t n = ny * nx;
+  if (n == 0) return RLIC_OK;  // nothing to compute (the reference loops zero times)
+  if (ny > INT32_MAX || nx > INT32_MAX || n > INT32_MAX || klen > INT32_MAX)
+    return set_error(RLIC_ERR_INVALID_ARGUMENT,
+                     "image of %lld x %lld pixels exceeds the supported 2^31-1 pixels",
+                     static_cast<long long>(ny), static_cast<long long>(nx));
+  if (!tex || !u || !v || !out)
+    return set_error(RLIC_ERR_INVALID_ARGUMENT, "null array pointer");
+  if (iterations > 0 && klen == 0)
+    return set_error(RLIC_ERR_EMPTY_KERNEL,
+                     "kernel is empty: kernel[len(kernel) // 2] does not exist");
+  if (klen > 0 && !kernel) return set_error(RLIC_ERR_INVALID_ARGUMENT, "null kernel pointer");
+
+  int dev = 0;
+  RLIC_HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+  configure_pool(dev);
+  hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+  const rlic::FieldGeom g{static_cast<int32_t>(ny), static_cast<int32_t>(nx), xl, xr, yl, yr};
+
+  if (location == RLIC_DEVICE) {
+    return run_on_device<T>(tex, tsy, tsx, u, usy, usx, v, vsy, vsx, static_cast<int32_t>(ny),
+                            static_cast<int32_t>(nx), kernel, static_cast<int32_t>(klen),
+                            uv_mode, g, iterations, out, profile, stream);
+  }
+
+  // Host arrays: upload the memory extent of each strided view, run, download.
+  DevMem d_tex, d_u, d_v, d_k, d_out;
+  const T* dev_in[3];
+  const T* host_in[3] = {tex, u, v};
+  const int64_t strides[3][2] = {{tsy, tsx}, {usy, usx}, {vsy, vsx}};
+  DevMem* bufs[3] = {&d_tex, &d_u, &d_v};
+  for (int a = 0; a < 3; ++a) {
+    int64_t lo, hi;
+    view_span(ny, nx, strides[a][0], strides[a][1], lo, hi);
+    const size_t count = static_cast<size_t>(hi - lo + 1);
+    RLIC_HIP_TRY(bufs[a]->alloc(sizeof(T) * count, stream), "allocating input");
+    RLIC_HIP_TRY(hipMemcpyAsync(bufs[a]->ptr, host_in[a] + lo, sizeof(T) * count,
+                                hipMemcpyHostToDevice, stream),
+                 "uploading input");
+    dev_in[a] = static_cast<const T*>(bufs[a]->ptr) - lo;
+  }
+  RLIC_HIP_TRY(d_k.alloc(sizeof(T) * (klen > 0 ? klen : 1), stream), "allocating kernel");
+  if (klen > 0) {
+    RLIC_HIP_TRY(hipMemcpyAsync(d_k.ptr, kernel, sizeof(T) * klen, hipMemcpyHostToDevice, stream),
+                 "uploading kernel");
+  }
+  RLIC_HIP_TRY(d_out.alloc(sizeof(T) * n, stream), "allocating output");
+  const int rc = run_on_device<T>(dev_in[0], tsy, tsx, dev_in[1], usy, usx, dev_in[2], vsy, vsx,
+                                  static_cast<int32_t>(ny), static_cast<int32_t>(nx),
+                                  static_cast<const T*>(d_k.ptr), static_cast<int32_t>(klen),
+                                  uv_mode, g, iterations, static_cast<T*>(d_out.ptr), profile,
+                                  stream);
+  if (rc != RLIC_OK) {
+    (void)hipStreamSynchronize(stream);
+    return rc;
+  }
+  RLIC_HIP_TRY(hipMemcpyAsync(out, d_out.ptr, sizeof(T) * n, hipMemcpyDeviceToHost, stream),
+               "downloading output");
+  RLIC_HIP_TRY(hipStreamSynchronize(stream), "synchronizing");
+  return RLIC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rlic_convolve_f32(const float* texture, int64_t tex_sy, int64_t tex_sx, const float* u,
+                      int64_t u_sy, int64_t u_sx, const float* v, int64_t v_sy, int64_t v_sx,
+                      int64_t ny, int64_t nx, const float* kernel, int64_t kernel_len,
+                      int uv_mode, int x_left, int x_right, int y_left, int y_right,
+                      int64_t iterations, float* out, int location, int profile, void* stream) {
+  return convolve_entry<float>(texture, tex_sy, tex_sx, u, u_sy, u_sx, v, v_sy, v_sx, ny, nx,
+                               kernel, kernel_len, uv_mode, x_left, x_right, y_left, y_right,
+                               iterations, out, location, profile, stream);
+}
+
+int rlic_convolve_f64(const double* texture, int64_t tex_sy, int64_t tex_sx, const double* u,
+                      int64_t u_sy, int64_t u_sx, const double* v, int64_t v_sy, int64_t v_sx,
+                      int64_t ny, int64_t nx, const double* kernel, int64_t kernel_len,
+                      int uv_mode, int x_left, int x_right, int y_left, int y_right,
+                      int64_t iterations, double* out, int location, int profile,
+                      void* stream) {
+  return convolve_entry<double>(texture, tex_sy, tex_sx, u, u_sy, u_sx, v, v_sy, v_sx, ny, nx,
+                                kernel, kernel_len, uv_mode, x_left, x_right, y_left, y_right,
+                                iterations, out, location, profile, stream);
+}
+
+const char* rlic_last_error(void) { return t_last_error.c_str(); }
+
+int rlic_profiler_enable(int on) {
+  std::lock_guard<std::mutex> lock(g_prof_mu);
+  g_prof_on = on != 0;
+  return RLIC_OK;
+}
+
+int64_t rlic_profiler_read(char* buf, int64_t len) {
+  std::lock_guard<std::mutex> lock(g_prof_mu);
+  for (const TimedLaunch& t : g_prof_pending) {
+    float ms = 0.f;
+    if (hipEventSynchronize(t.stop) == hipSuccess &&
+        hipEventElapsedTime(&ms, t.start, t.stop) == hipSuccess) {
+      auto& agg = g_prof_done[t.name];
+      agg.first += 1;
+      agg.second += ms;
+    }
+    (void)hipEventDestroy(t.start);
+    (void)hipEventDestroy(t.stop);
+  }
+  g_prof_pending.clear();
+  std::string json = "{";
+  bool first = true;
+  for (const auto& kv : g_prof_done) {
+    char item[256];
+    snprintf(item, sizeof(item), "%s\"%s\": {\"count\": %lld, \"total_ms\": %.6f}",
+             first ? "" : ", ", kv.first.c_str(), static_cast<long long>(kv.second.first),
+             kv.second.second);
+    json += item;
+    first = false;
+  }
+  json += "}";
+  const int64_t need = static_cast<int64_t>(json.size()) + 1;
+  if (buf && len >= need) {
+    std::memcpy(buf, json.c_str(), need);
+    g_prof_done.clear();  // handed over: start a new record
+  }
+  return need;
+}
+
+const char* rlic_version(void) { return "0.1.0"; }
+
+}  // extern "C"

@@ -1,0 +1,32 @@
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+
+def pytest_configure(config: pytest.Config) -> None:
+    config.addinivalue_line("markers", "gpu: needs a ROCm GPU (MI355X / gfx950)")
+
+
+def _gpu_available() -> bool:
+    try:
+        import torch
+    except ImportError:
+        return False
+    return bool(torch.cuda.is_available())
+
+
+@pytest.fixture(scope="session")
+def gpu_device():
+    """The ROCm device the GPU tests run on; fails (not skips) without one."""
+    import torch
+
+    if not _gpu_available():
+        pytest.fail("GPU test selected but no ROCm device is visible")
+    return torch.device("cuda:0")

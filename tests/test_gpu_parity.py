@@ -1,0 +1,229 @@
+"""GPU parity: the HIP path against the oracle, bit for bit.
+
+Every case runs the product path (``librlic_hip.so`` through the C ABI) on
+the GPU and compares with the CPU oracle on identical inputs.  Both fp64
+and fp32 are required to match exactly: the kernels use the reference's
+operations in the reference's order with IEEE rounding, so there is no
+tolerance to grant.
+"""
+
+from __future__ import annotations
+
+import itertools
+import os
+
+import numpy as np
+import pytest
+
+import rlic_amd
+from oracle import cpu
+from rlic_amd import _core
+
+pytestmark = pytest.mark.gpu
+
+PROFILES = [0, 1, 2, 3]
+MODES = ["velocity", "polarization"]
+BOUNDS = [
+    "closed",
+    "periodic",
+    {"x": "periodic", "y": "closed"},
+    {"x": ("closed", "closed"), "y": "periodic"},
+]
+
+
+def _fields(rng, shape, dtype, kind="random"):
+    ny, nx = shape
+    tex = rng.random(shape).astype(dtype)
+    if kind == "random":
+        u = (rng.random(shape) - 0.5).astype(dtype)
+        v = (rng.random(shape) - 0.5).astype(dtype)
+    elif kind == "holes":  # NaN holes, exact zeros, signed zeros
+        u = (rng.random(shape) - 0.5).astype(dtype)
+        v = (rng.random(shape) - 0.5).astype(dtype)
+        u[rng.random(shape) < 0.05] = np.nan
+        v[rng.random(shape) < 0.05] = np.nan
+        zero = rng.random(shape) < 0.1
+        u[zero] = 0.0
+        v[zero] = -0.0
+        v[:, nx // 3] = 0.0
+        u[ny // 2, :] = -0.0
+    elif kind == "ties":  # unit half-plane fields: exact tx == ty ties
+        jj = np.broadcast_to(np.arange(nx), shape)
+        ii = np.broadcast_to(np.arange(ny)[:, None], shape)
+        u = np.where(jj < nx / 2, -1.0, 1.0).astype(dtype)
+        v = np.where(ii < ny / 2, -1.0, 1.0).astype(dtype)
+    else:
+        raise ValueError(kind)
+    return tex, u, v
+
+
+def _native(dtype):
+    return _core.convolve_f32 if np.dtype(dtype) == np.float32 else _core.convolve_f64
+
+
+def _bpair(b):
+    if isinstance(b, str):
+        return ((b, b), (b, b))
+    bx = (b["x"], b["x"]) if isinstance(b["x"], str) else tuple(b["x"])
+    by = (b["y"], b["y"]) if isinstance(b["y"], str) else tuple(b["y"])
+    return (bx, by)
+
+
+def _assert_same(got, expected, label=""):
+    assert got.dtype == expected.dtype and got.shape == expected.shape
+    if not np.array_equal(got, expected, equal_nan=True):
+        diff = ~((got == expected) | (np.isnan(got) & np.isnan(expected)))
+        n = int(np.count_nonzero(diff))
+        i, j = np.argwhere(diff)[0]
+        raise AssertionError(
+            f"{label}: {n}/{got.size} pixels differ; first at ({i},{j}): "
+            f"{got[i, j]!r} vs oracle {expected[i, j]!r}"
+        )
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("profile", PROFILES)
+@pytest.mark.parametrize("uv_mode", MODES)
+def test_random_fields_all_profiles(dtype, profile, uv_mode):
+    rng = np.random.default_rng(100 + profile)
+    for b, (klen, iters) in zip(BOUNDS, [(11, 1), (12, 2), (33, 3), (5, 1)]):
+        tex, u, v = _fields(rng, (37, 53), dtype)
+        k = rng.random(klen).astype(dtype)
+        exp = cpu.convolve(tex, u, v, kernel=k, uv_mode=uv_mode, boundaries=b,
+                           iterations=iters, profile=profile)
+        got = _native(dtype)(tex, (u, v, uv_mode), k, _bpair(b), iters, profile=profile)
+        _assert_same(got, exp, f"{dtype.__name__} p{profile} {uv_mode} {b} L={klen} n={iters}")
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("kind", ["holes", "ties"])
+@pytest.mark.parametrize("profile", [1, 3])
+def test_special_fields(dtype, kind, profile):
+    rng = np.random.default_rng(7)
+    for uv_mode, b in itertools.product(MODES, BOUNDS[:3]):
+        tex, u, v = _fields(rng, (31, 33), dtype, kind)
+        k = np.sin(np.arange(31) * np.pi / 31).astype(dtype)
+        exp = cpu.convolve(tex, u, v, kernel=k, uv_mode=uv_mode, boundaries=b,
+                           iterations=2, profile=profile)
+        got = _native(dtype)(tex, (u, v, uv_mode), k, _bpair(b), 2, profile=profile)
+        _assert_same(got, exp, f"{kind} {uv_mode} {b}")
+
+
+@pytest.mark.parametrize("klen", [1, 2, 3, 4, 5, 64, 65, 129])
+def test_kernel_lengths(klen):
+    rng = np.random.default_rng(klen)
+    tex, u, v = _fields(rng, (40, 24), np.float64)
+    k = rng.random(klen)
+    exp = cpu.convolve(tex, u, v, kernel=k, uv_mode="polarization", boundaries="periodic",
+                       iterations=2)
+    got = rlic_amd.convolve(tex, u, v, kernel=k, uv_mode="polarization",
+                            boundaries="periodic", iterations=2)
+    _assert_same(got, exp, f"L={klen}")
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (1, 17), (17, 1), (2, 3), (16, 16), (17, 15), (5, 300)])
+def test_ragged_shapes(shape):
+    rng = np.random.default_rng(sum(shape))
+    for dtype in (np.float32, np.float64):
+        tex, u, v = _fields(rng, shape, dtype)
+        k = rng.random(9).astype(dtype)
+        for b in ("closed", "periodic"):
+            exp = cpu.convolve(tex, u, v, kernel=k, boundaries=b, iterations=3)
+            got = rlic_amd.convolve(tex, u, v, kernel=k, boundaries=b, iterations=3)
+            _assert_same(got, exp, f"{shape} {b}")
+
+
+def test_empty_images():
+    for shape in [(0, 5), (5, 0), (0, 0)]:
+        tex = np.zeros(shape)
+        out = rlic_amd.convolve(tex, tex, tex, kernel=np.ones(3))
+        assert out.shape == shape
+        # with nothing to compute the reference never reads the kernel
+        out = _core.convolve_f64(tex, (tex, tex, "velocity"), np.ones(0),
+                                 (("closed",) * 2,) * 2, 2)
+        assert out.shape == shape
+
+
+def test_empty_kernel_raises():
+    tex = np.ones((4, 4))
+    with pytest.raises(ValueError, match="kernel is empty"):
+        rlic_amd.convolve(tex, tex, tex, kernel=np.ones(0))
+
+
+def test_strided_inputs():
+    """Transposed, sliced, negative-stride and broadcast views (any strides)."""
+    rng = np.random.default_rng(3)
+    big = rng.random((80, 90))
+    tex = big[5:45, 10:70:2]  # sliced and strided
+    u = (rng.random((30, 40)) - 0.5).T  # F-order
+    v = (rng.random((40, 30))[::-1] - 0.5)  # negative row stride
+    k = rng.random(13)
+    exp = cpu.convolve(np.ascontiguousarray(tex), np.ascontiguousarray(u),
+                       np.ascontiguousarray(v), kernel=k, iterations=2)
+    got = rlic_amd.convolve(tex, u, v, kernel=k, iterations=2)
+    _assert_same(got, exp, "strided")
+    ub = np.broadcast_to(rng.random(30) - 0.5, (40, 30))
+    vb = np.broadcast_to((rng.random(40) - 0.5)[:, None], (40, 30))
+    exp = cpu.convolve(np.ascontiguousarray(tex), np.ascontiguousarray(ub),
+                       np.ascontiguousarray(vb), kernel=k, boundaries="periodic")
+    got = rlic_amd.convolve(tex, ub, vb, kernel=k, boundaries="periodic")
+    _assert_same(got, exp, "broadcast")
+
+
+def test_retrace_path_matches_replay():
+    """Iterations re-traced every pass (no move record) give the same bits."""
+    rng = np.random.default_rng(11)
+    tex, u, v = _fields(rng, (50, 61), np.float32, "holes")
+    k = rng.random(21).astype(np.float32)
+    exp = cpu.convolve(tex, u, v, kernel=k, uv_mode="polarization", iterations=4)
+    got_replay = rlic_amd.convolve(tex, u, v, kernel=k, uv_mode="polarization", iterations=4)
+    os.environ["RLIC_REPLAY"] = "0"
+    try:
+        got_retrace = rlic_amd.convolve(tex, u, v, kernel=k, uv_mode="polarization",
+                                        iterations=4)
+    finally:
+        del os.environ["RLIC_REPLAY"]
+    _assert_same(got_replay, exp, "replay")
+    _assert_same(got_retrace, exp, "retrace")
+
+
+def test_readme_example_config():
+    """The reference README example: 256x256 float64, L=65, periodic, 1 iteration."""
+    shape = (256, 256)
+    texture = np.random.default_rng(0).random(shape)
+    x = np.linspace(0, np.pi, 256)
+    U = np.broadcast_to(np.cos(2 * x), shape)
+    V = np.broadcast_to(np.sin(x).T, shape)
+    kernel = 1 - np.abs(np.linspace(-1, 1, 65))
+    exp = cpu.convolve(texture, U, V, kernel=kernel, boundaries="periodic")
+    for n in (1, 5):
+        got = rlic_amd.convolve(texture, U, V, kernel=kernel, boundaries="periodic",
+                                iterations=n)
+        if n == 1:
+            _assert_same(got, exp, "README example")
+        else:
+            _assert_same(got, cpu.convolve(texture, U, V, kernel=kernel,
+                                           boundaries="periodic", iterations=n), "n=5")
+
+
+def test_torch_device_tensors(gpu_device):
+    import torch
+
+    rng = np.random.default_rng(5)
+    for dtype in (np.float32, np.float64):
+        tex, u, v = _fields(rng, (45, 70), dtype, "holes")
+        k = rng.random(17).astype(dtype)
+        exp = cpu.convolve(tex, u, v, kernel=k, uv_mode="polarization",
+                           boundaries={"x": "periodic", "y": "closed"}, iterations=3)
+        t = [torch.from_numpy(np.ascontiguousarray(a)).to(gpu_device) for a in (tex, u, v, k)]
+        got = rlic_amd.convolve(t[0], t[1], t[2], kernel=t[3], uv_mode="polarization",
+                                boundaries={"x": "periodic", "y": "closed"}, iterations=3)
+        assert got.device == gpu_device
+        _assert_same(got.cpu().numpy(), exp, f"torch {dtype.__name__}")
+        # transposed device views are read in place
+        got_t = rlic_amd.convolve(t[0].T, t[2].T, t[1].T, kernel=t[3],
+                                  boundaries={"x": "closed", "y": "periodic"}, iterations=1)
+        exp_t = cpu.convolve(np.ascontiguousarray(tex.T), np.ascontiguousarray(v.T),
+                             np.ascontiguousarray(u.T), kernel=k,
+                             boundaries={"x": "closed", "y": "periodic"})
+        _assert_same(got_t.cpu().numpy(), exp_t, "torch transposed")
