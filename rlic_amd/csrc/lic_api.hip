@@ -149,6 +149,16 @@ int run_on_device(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
     return RLIC_OK;
   }
   const int64_t n = static_cast<int64_t>(ny) * nx;
+  // kernel weights with zero padding on both sides
+  DevMem kpad_mem;
+  const int32_t pad = rlic::kernel_pad();
+  RLIC_HIP_TRY(kpad_mem.alloc(sizeof(T) * (klen + 2 * pad), stream), "allocating kernel copy");
+  {
+    LaunchTimer timer("lic_pad_kernel", stream);
+    RLIC_HIP_TRY(rlic::launch_pad_kernel<T>(kernel, klen, static_cast<T*>(kpad_mem.ptr), stream),
+                 "kernel copy");
+  }
+  kernel = static_cast<const T*>(kpad_mem.ptr) + pad;
   DevMem cells;
   RLIC_HIP_TRY(cells.alloc(sizeof(rlic::Cell<T>) * n, stream), "allocating packed cells");
   auto* c = static_cast<rlic::Cell<T>*>(cells.ptr);
@@ -158,7 +168,8 @@ int run_on_device(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
         rlic::launch_pack<T>(tex, tsy, tsx, u, usy, usx, v, vsy, vsx, ny, nx, c, stream),
         "pack");
   }
-  const rlic::TileMap m = rlic::make_tile_map(ny, nx, env_int("RLIC_TILE_BAND", 16));
+  const rlic::TileMap m = rlic::make_tile_map(ny, nx, env_int("RLIC_TILE_BAND", 16),
+                                              env_int("RLIC_WAVE_W_LOG2", 3));
   if (iterations == 1) {
     LaunchTimer timer("lic_trace", stream);
     RLIC_HIP_TRY(rlic::launch_trace<T>(c, kernel, klen, g, m, uv_mode, profile, out, nullptr,
@@ -170,9 +181,9 @@ int run_on_device(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
   RLIC_HIP_TRY(tmp_mem.alloc(sizeof(T) * n, stream), "allocating iteration buffer");
   T* tmp = static_cast<T*>(tmp_mem.ptr);
 
-  // Move record: code_words(klen) words per pixel slot.
-  const size_t code_bytes = sizeof(uint32_t) * static_cast<size_t>(rlic::code_words(klen)) *
-                            static_cast<size_t>(m.n_tiles) * rlic::kBlock;
+  // Move record: code_chunks(klen) 16-byte chunks per pixel.
+  const size_t code_bytes =
+      sizeof(uint4) * static_cast<size_t>(rlic::code_chunks(klen)) * static_cast<size_t>(n);
   bool replay = env_int("RLIC_REPLAY", 1) != 0;
   if (replay) {
     size_t free_b = 0, total_b = 0;
@@ -186,7 +197,9 @@ int run_on_device(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
     replay = false;
   }
   if (replay) {
-    auto* codes = static_cast<uint32_t*>(code_mem.ptr);
+    auto* codes = static_cast<uint4*>(code_mem.ptr);
+    const int lds_policy = env_int("RLIC_REPLAY_LDS", 1);  // 0 off, 1 when it fits
+    const bool use_lds = lds_policy != 0 && rlic::lds_replay_fits<T>(klen);
     // pass `it` (1-based) writes A when odd, B when even; the last lands in out
     T* a = ((iterations - 1) % 2 == 0) ? out : tmp;
     T* b = (a == out) ? tmp : out;
@@ -199,9 +212,16 @@ int run_on_device(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
     for (int64_t it = 2; it <= iterations; ++it) {
       const T* src = (it % 2 == 0) ? a : b;
       T* dst = (it % 2 == 0) ? b : a;
-      LaunchTimer timer("lic_replay", stream);
-      RLIC_HIP_TRY(rlic::launch_replay<T>(src, kernel, klen, g, m, profile, dst, codes, stream),
-                   "replay");
+      if (use_lds) {
+        LaunchTimer timer("lic_replay_lds", stream);
+        RLIC_HIP_TRY(
+            rlic::launch_replay_lds<T>(src, kernel, klen, g, profile, dst, codes, stream),
+            "replay");
+      } else {
+        LaunchTimer timer("lic_replay", stream);
+        RLIC_HIP_TRY(rlic::launch_replay<T>(src, kernel, klen, g, m, profile, dst, codes, stream),
+                     "replay");
+      }
     }
     return RLIC_OK;
   }

@@ -40,18 +40,91 @@ struct CellVal {
 };
 
 __device__ __forceinline__ CellVal<float> load_cell(const Cell<float>* __restrict__ p,
-                                                    int32_t idx) {
+                                                    uint32_t idx) {
   const float4 c = *reinterpret_cast<const float4*>(p + idx);
   return {c.x, c.y, c.z};
 }
 
 __device__ __forceinline__ CellVal<double> load_cell(const Cell<double>* __restrict__ p,
-                                                     int32_t idx) {
+                                                     uint32_t idx) {
   const double2* q = reinterpret_cast<const double2*>(p + idx);
   const double2 a = q[0];
   const double2 b = q[1];
   return {a.x, a.y, b.x};
 }
+
+// ---------------------------------------------------------------------------
+// Gathers.  With BUF, arrays are addressed through a raw buffer resource and
+// a 32-bit byte offset (one VALU add per address, hardware range check: an
+// out-of-range read returns 0 instead of faulting); usable when the array
+// spans fewer than 2^32 bytes.  Otherwise 64-bit global addresses are used.
+// ---------------------------------------------------------------------------
+// gfx9 raw-buffer descriptor word 3 (no swizzle, dword data format).
+constexpr int32_t kBufferWord3 = 0x00020000;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
+                                           kBufferWord3);
+}
+
+__device__ __forceinline__ float buf_load(const __amdgpu_buffer_rsrc_t r, uint32_t off, float*) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ double buf_load(const __amdgpu_buffer_rsrc_t r, uint32_t off,
+                                           double*) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+
+// The b64/b128 builtins return GCC vector_size vectors of unsigned int;
+// they are bit-cast whole (bit-casting a subscripted lane miscompiles).
+__device__ __forceinline__ CellVal<float> buf_load_cell(const __amdgpu_buffer_rsrc_t r,
+                                                        uint32_t off, float*) {
+  const float4 c = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+  return {c.x, c.y, c.z};
+}
+__device__ __forceinline__ CellVal<double> buf_load_cell(const __amdgpu_buffer_rsrc_t r,
+                                                         uint32_t off, double*) {
+  const double2 uv =
+      __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+  const double x = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off + 16u, 0, 0));
+  return {uv.x, uv.y, x};
+}
+
+// Cell gather at `pos`: a byte offset (BUF) or an element index (global).
+template <typename T, bool BUF>
+struct CellGather {
+  const Cell<T>* base;
+  __amdgpu_buffer_rsrc_t rsrc;
+  __device__ __forceinline__ CellGather(const Cell<T>* p, int64_t n) : base(p) {
+    if constexpr (BUF) rsrc = make_rsrc(p, static_cast<uint32_t>(n * sizeof(Cell<T>)));
+  }
+  static constexpr uint32_t unit = BUF ? static_cast<uint32_t>(sizeof(Cell<T>)) : 1u;
+  __device__ __forceinline__ CellVal<T> operator()(uint32_t pos) const {
+    if constexpr (BUF) {
+      return buf_load_cell(rsrc, pos, static_cast<T*>(nullptr));
+    } else {
+      return load_cell(base, pos);
+    }
+  }
+};
+
+// Texture gather at `pos`: a byte offset (BUF) or an element index (global).
+template <typename T, bool BUF>
+struct ValueGather {
+  const T* base;
+  __amdgpu_buffer_rsrc_t rsrc;
+  __device__ __forceinline__ ValueGather(const T* p, int64_t n) : base(p) {
+    if constexpr (BUF) rsrc = make_rsrc(p, static_cast<uint32_t>(n * sizeof(T)));
+  }
+  static constexpr uint32_t unit = BUF ? static_cast<uint32_t>(sizeof(T)) : 1u;
+  __device__ __forceinline__ T operator()(uint32_t pos) const {
+    if constexpr (BUF) {
+      return buf_load(rsrc, pos, static_cast<T*>(nullptr));
+    } else {
+      return base[pos];
+    }
+  }
+};
 
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
@@ -93,34 +166,94 @@ __device__ __forceinline__ T time_to_next_pixel(T velocity, T frac) {
   }
 }
 
-// Boundary rule applied to one coordinate after each step
-// (src/lib.rs:83-95): -1 is the "stepped off the low edge" sentinel.
-__device__ __forceinline__ int32_t apply_bound(int32_t c, int32_t size, int32_t low_periodic,
-                                               int32_t high_periodic) {
-  if (c == -1) return low_periodic ? size - 1 : 0;
-  if (c == size) return high_periodic ? 0 : size - 1;
-  return c;
-}
-
 struct FieldGeom {
   int32_t ny, nx;
   int32_t x_left, x_right, y_left, y_right;  // 1 = periodic, 0 = closed
 };
 
-// Block -> 16x16 pixel tile.  Workgroups are dealt round-robin to the 8
-// XCDs; each XCD is given a contiguous run of logical tiles, and logical
+// Per-field constants of the incremental step rule.  A position is kept as
+// (row_off, j) with row_off = i * nx, so the cell index is row_off + j and a
+// step changes one of the two by +-1 / +-nx, or jumps to the target of the
+// edge it crossed (src/lib.rs:83-95: closed clamps to the edge cell,
+// periodic wraps to the opposite edge).
+struct StepGeom {
+  uint32_t nx, nxm1, last_row;        // last_row = (ny - 1) * nx
+  uint32_t x_lo_target, x_hi_target;  // j after stepping off the left / right edge
+  uint32_t y_lo_target, y_hi_target;  // row_off after stepping off the top / bottom edge
+  uint32_t code_x_lo, code_x_hi;     // move code recorded for such a step
+  uint32_t code_y_lo, code_y_hi;
+};
+
+// Move codes recorded by the tracing pass (4 bits per step, 8 per word).
+// A code names the cell-index change of one step with the boundary rule
+// already applied, so replay is a table lookup: a clamped step is a stay.
+enum : uint32_t {
+  kMoveStay = 0,
+  kMoveXPlus = 1,
+  kMoveXMinus = 2,
+  kMoveYPlus = 3,
+  kMoveYMinus = 4,
+  kMoveXWrapHi = 5,  // j: nx-1 -> 0
+  kMoveXWrapLo = 6,  // j: 0 -> nx-1
+  kMoveYWrapHi = 7,  // i: ny-1 -> 0
+  kMoveYWrapLo = 8,  // i: 0 -> ny-1
+  kMoveStop = 15     // streamline ended on a NaN vector (src/lib.rs:336-338)
+};
+
+// Positions are in `unit`s: 1 for element indices, the element size for
+// byte offsets.
+__host__ __device__ inline StepGeom make_step_geom(const FieldGeom& g, uint32_t unit) {
+  StepGeom s;
+  s.nx = static_cast<uint32_t>(g.nx) * unit;
+  s.nxm1 = static_cast<uint32_t>(g.nx - 1) * unit;
+  s.last_row = static_cast<uint32_t>(g.ny - 1) * static_cast<uint32_t>(g.nx) * unit;
+  s.x_lo_target = g.x_left ? s.nxm1 : 0;
+  s.x_hi_target = g.x_right ? 0 : s.nxm1;
+  s.y_lo_target = g.y_left ? s.last_row : 0;
+  s.y_hi_target = g.y_right ? 0 : s.last_row;
+  s.code_x_lo = g.x_left ? kMoveXWrapLo : kMoveStay;
+  s.code_x_hi = g.x_right ? kMoveXWrapHi : kMoveStay;
+  s.code_y_lo = g.y_left ? kMoveYWrapLo : kMoveStay;
+  s.code_y_hi = g.y_right ? kMoveYWrapHi : kMoveStay;
+  return s;
+}
+
+// Position change of each move code in `unit`s (replay's lookup table).
+// Deltas are added modulo 2^32 (positions are uint32_t).
+__host__ __device__ inline uint32_t move_delta(uint32_t code, const FieldGeom& g, uint32_t unit) {
+  const uint32_t nx = static_cast<uint32_t>(g.nx), ny = static_cast<uint32_t>(g.ny);
+  switch (code) {
+    case kMoveXPlus: return unit;
+    case kMoveXMinus: return 0u - unit;
+    case kMoveYPlus: return nx * unit;
+    case kMoveYMinus: return 0u - nx * unit;
+    case kMoveXWrapHi: return 0u - (nx - 1) * unit;
+    case kMoveXWrapLo: return (nx - 1) * unit;
+    case kMoveYWrapHi: return 0u - (ny - 1) * nx * unit;
+    case kMoveYWrapLo: return (ny - 1) * nx * unit;
+    default: return 0u;
+  }
+}
+
+// Block -> pixel tile.  A block is 4 wave64s in a 2x2 arrangement; each wave
+// covers a wave_w x (64 / wave_w) rectangle, so a block tile is
+// (2 * wave_w) x (128 / wave_w).  Workgroups are dealt round-robin to the
+// 8 XCDs; each XCD is given a contiguous run of logical tiles, and logical
 // tiles run column-major inside horizontal strips `band` tiles tall, so the
-// tiles resident on one XCD at a time form a compact square whose
-// streamline halo stays in that XCD's L2.
+// tiles resident on one XCD at a time form a compact patch whose streamline
+// halo stays in that XCD's L2.
 struct TileMap {
   int32_t tiles_x, tiles_y, n_tiles;
   int32_t per_xcd, rem_xcd;  // n_tiles = 8 * per_xcd + rem_xcd
   int32_t band;
+  int32_t wave_w_log2;       // 3, 4 or 5 (wave tile 8x8, 16x4, 32x2)
 };
 
-constexpr int kTile = 16;
-constexpr int kBlock = kTile * kTile;
+constexpr int kBlock = 256;
 constexpr int kXcds = 8;
+
+__host__ __device__ inline int32_t tile_w(const TileMap& m) { return 2 << m.wave_w_log2; }
+__host__ __device__ inline int32_t tile_h(const TileMap& m) { return 2 << (6 - m.wave_w_log2); }
 
 __device__ __forceinline__ void tile_of_block(int32_t b, const TileMap& m, int32_t& ty,
                                               int32_t& tx) {
@@ -135,21 +268,13 @@ __device__ __forceinline__ void tile_of_block(int32_t b, const TileMap& m, int32
   ty = strip * m.band + (within - tx * h);
 }
 
-// Thread -> pixel inside the tile: each wave64 covers an 8x8 square.
-__device__ __forceinline__ void pixel_of_thread(int32_t tid, int32_t& py, int32_t& px) {
+// Thread -> pixel (py, px) inside the block tile.
+__device__ __forceinline__ void pixel_of_thread(int32_t tid, const TileMap& m, int32_t& py,
+                                                int32_t& px) {
   const int32_t w = tid >> 6, l = tid & 63;
-  px = ((w & 1) << 3) | (l & 7);
-  py = ((w >> 1) << 3) | (l >> 3);
+  const int32_t ww = m.wave_w_log2, wh = 6 - m.wave_w_log2;
+  px = ((w & 1) << ww) | (l & ((1 << ww) - 1));
+  py = ((w >> 1) << wh) | (l >> ww);
 }
-
-// Move codes recorded by the tracing pass (4 bits per step, 8 per word).
-enum : uint32_t {
-  kMoveStay = 0,
-  kMoveXPlus = 1,
-  kMoveXMinus = 2,
-  kMoveYPlus = 3,
-  kMoveYMinus = 4,
-  kMoveStop = 7
-};
 
 }  // namespace rlic

@@ -8,7 +8,8 @@
 //               iterations (src/lib.rs:416-420, 432-440), so every iteration
 //               visits the same cells; replay walks the recorded moves and
 //               only gathers the new texture, in the same summation order
-//   lic_copy2d  strided 2-D copy (iterations == 0 and texture re-packing)
+//   lic_copy2d  strided 2-D copy (iterations == 0)
+//   lic_repack_texture  texture slot refresh when passes are re-traced
 #include "lic_device.h"
 #include "lic_launch.h"
 
@@ -32,13 +33,23 @@ __global__ __launch_bounds__(256) void lic_pack(const T* __restrict__ tex, int64
   cells[static_cast<int64_t>(i) * nx + j] = c;
 }
 
-// Overwrite the texture slot of the packed cells with a C-order array.
 template <typename T>
 __global__ __launch_bounds__(256) void lic_repack_texture(const T* __restrict__ x, int32_t n,
                                                           Cell<T>* __restrict__ cells) {
   const int32_t p = blockIdx.x * 256 + threadIdx.x;
   if (p >= n) return;
   cells[p].x = x[p];
+}
+
+// dst[0 .. klen + 2*pad) = [pad zeros][kernel][pad zeros]
+template <typename T>
+__global__ __launch_bounds__(256) void lic_pad_kernel(const T* __restrict__ kernel, int32_t klen,
+                                                      int32_t pad, T* __restrict__ dst) {
+  const int32_t total = klen + 2 * pad;
+  for (int32_t t = blockIdx.x * 256 + threadIdx.x; t < total; t += gridDim.x * 256) {
+    const int32_t k = t - pad;
+    dst[t] = (k >= 0 && k < klen) ? kernel[k] : T(0);
+  }
 }
 
 template <typename T>
@@ -51,163 +62,423 @@ __global__ __launch_bounds__(256) void lic_copy2d(const T* __restrict__ src, int
   dst[static_cast<int64_t>(i) * nx + j] = src[i * sy + j * sx];
 }
 
-// One streamline direction of one pixel (src/lib.rs:304-362), starting with
-// the vector (pu, pv) of the start cell already loaded.  Returns the updated
-// accumulator.  With REC, writes the move codes of this direction to
-// codes[(w0 + s/8) * np + p].
-template <typename T, bool FMA, bool BRANCHLESS, bool POL, bool FORWARD, bool REC>
-__device__ __forceinline__ T trace_direction(T acc, T pu, T pv, int32_t i, int32_t j,
-                                             const Cell<T>* __restrict__ cells,
+// Kernel weights are read through a copy padded with kKernelPad zeros on
+// each side, so a group of 8 taps can be loaded without bounds checks.
+constexpr int32_t kKernelPad = 8;
+
+// Move-code layout: a direction's steps are cut into chunks of 32 steps
+// (4 words of 8 four-bit codes); chunk c of pixel p (row-major index
+// i * nx + j) is the 16-byte vector codes4[(c0 + c) * ny * nx + p], so one
+// lane moves 32 steps of codes with one load or store, and the tracing and
+// replay kernels may tile the image differently.
+constexpr int32_t kChunkSteps = 32;
+
+__host__ __device__ inline int32_t chunks_of(int32_t nsteps) {
+  return (nsteps + kChunkSteps - 1) / kChunkSteps;
+}
+
+// One streamline direction of one pixel (src/lib.rs:304-362), starting from
+// the vector (pu, pv) of the start cell, already loaded.  The position is
+// (row, j) with row = i * nx, both in the gather's units.  Returns the
+// updated accumulator.  With REC, writes the move code of every step.
+//
+// The step is written without divergent branches: a lane whose line has
+// ended (NaN vector) keeps executing with its position frozen and its
+// accumulator untouched, and a lane with a zero vector keeps its position
+// (src/lib.rs:242-244); steps run in groups of 8 (one code word, one batch
+// of 8 weights) and a wave leaves the loop once all its lines have ended.
+template <typename T, bool FMA, bool BRANCHLESS, bool POL, bool FORWARD, bool REC, bool BUF>
+__device__ __forceinline__ T trace_direction(T acc, T pu, T pv, uint32_t row, uint32_t j,
+                                             const CellGather<T, BUF>& cell,
                                              const T* __restrict__ kernel, int32_t kmid,
-                                             int32_t nsteps, const FieldGeom& g,
-                                             uint32_t* __restrict__ codes, int64_t np,
-                                             int64_t p, int32_t w0) {
-  T fx = T(0.5), fy = T(0.5);
-  T last_u = T(0), last_v = T(0);
-  uint32_t word = 0;
-  for (int32_t s = 0; s < nsteps; ++s) {
-    const int32_t k = FORWARD ? kmid + 1 + s : kmid - 1 - s;
-    if (isnan_t(pu) || isnan_t(pv)) {  // src/lib.rs:336-338
-      if constexpr (REC) {
-        word |= kMoveStop << (4 * (s & 7));
-        codes[(w0 + (s >> 3)) * np + p] = word;
-      }
-      break;
-    }
-    if constexpr (POL) {  // src/lib.rs:339-347, unfused dot product
-      const T d1 = pu * last_u;
-      const T d2 = pv * last_v;
-      if (d1 + d2 < T(0)) {
-        pu = -pu;
-        pv = -pv;
-      }
-      last_u = pu;
-      last_v = pv;
-    }
-    const T mu = FORWARD ? pu : -pu;  // src/lib.rs:348-351
-    const T mv = FORWARD ? pv : -pv;
-    uint32_t code = kMoveStay;
-    if (!(mu == T(0) && mv == T(0))) {  // src/lib.rs:242-244
-      const T tx = time_to_next_pixel<T, FMA, BRANCHLESS>(mu, fx);
-      const T ty = time_to_next_pixel<T, FMA, BRANCHLESS>(mv, fy);
-      if (tx < ty) {  // src/lib.rs:249-258, update_state :208-233
-        if (mu >= T(0)) {
-          j += 1;
-          fx = T(0);
-          code = kMoveXPlus;
-        } else {
-          j -= 1;
-          fx = T(1);
-          code = kMoveXMinus;
+                                             int32_t nsteps, const StepGeom& sg,
+                                             uint4* __restrict__ codes, int64_t np, int64_t p,
+                                             int32_t c0) {
+  T fx = T(0.5), fy = T(0.5);      // src/lib.rs:315-318
+  T last_u = T(0), last_v = T(0);  // src/lib.rs:320-323
+  bool alive = true;
+  const uint32_t unit = cell.unit;
+  const int32_t nchunks = chunks_of(nsteps);
+  for (int32_t ch = 0; ch < nchunks; ++ch) {
+    uint32_t words[4] = {0u, 0u, 0u, 0u};
+    bool any_alive = true;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const int32_t s0 = ch * kChunkSteps + 8 * g;
+      if (s0 >= nsteps || !any_alive) break;  // uniform
+      T kw[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        kw[q] = kernel[FORWARD ? kmid + 1 + s0 + q : kmid - 1 - s0 - q];
+      uint32_t word = 0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        if (s0 + q >= nsteps) break;  // uniform: last group only
+        // a NaN component ends the line (src/lib.rs:336-338)
+        const bool ends = __builtin_isunordered(pu, pv);
+        const bool live = alive && !ends;
+        if constexpr (POL) {  // src/lib.rs:339-347, unfused dot product
+          const T d1 = pu * last_u;
+          const T d2 = pv * last_v;
+          const bool flip = d1 + d2 < T(0);
+          pu = flip ? -pu : pu;
+          pv = flip ? -pv : pv;
+          last_u = pu;
+          last_v = pv;
         }
-        fy = macc<FMA>(tx, mv, fy);
-      } else {  // src/lib.rs:259-269
-        if (mv >= T(0)) {
-          i += 1;
-          fy = T(0);
-          code = kMoveYPlus;
-        } else {
-          i -= 1;
-          fy = T(1);
-          code = kMoveYMinus;
+        const T mu = FORWARD ? pu : -pu;  // src/lib.rs:348-351
+        const T mv = FORWARD ? pv : -pv;
+        const T tx = time_to_next_pixel<T, FMA, BRANCHLESS>(mu, fx);
+        const T ty = time_to_next_pixel<T, FMA, BRANCHLESS>(mv, fy);
+        // x edge first when tx < ty; ties and NaN go to y (src/lib.rs:249-269)
+        const bool xs = tx < ty;
+        // no move when both components are +-0 (src/lib.rs:242-244)
+        const bool moves = live && (mu != T(0) || mv != T(0));
+        // update_state (src/lib.rs:208-233) on the crossed axis
+        const T vpar = xs ? mu : mv;
+        const bool pos = vpar >= T(0);
+        const T frac_par = pos ? T(0) : T(1);
+        const T frac_orth = macc<FMA>(xs ? tx : ty, xs ? mv : mu, xs ? fy : fx);
+        fx = moves ? (xs ? frac_par : frac_orth) : fx;
+        fy = moves ? (xs ? frac_orth : frac_par) : fy;
+        // coordinate step with the boundary rule (src/lib.rs:83-95, 112-115)
+        const uint32_t c = xs ? j : row;
+        const uint32_t edge_at = xs ? (pos ? sg.nxm1 : 0u) : (pos ? sg.last_row : 0u);
+        const uint32_t stride = xs ? unit : sg.nx;
+        const uint32_t target = xs ? (pos ? sg.x_hi_target : sg.x_lo_target)
+                                   : (pos ? sg.y_hi_target : sg.y_lo_target);
+        const bool edge = c == edge_at;
+        const uint32_t moved = edge ? target : (pos ? c + stride : c - stride);
+        j = (moves && xs) ? moved : j;
+        row = (moves && !xs) ? moved : row;
+        const CellVal<T> cv = cell(row + j);
+        const T next_acc = macc<FMA>(kw[q], cv.x, acc);  // src/lib.rs:353-360
+        acc = live ? next_acc : acc;
+        pu = cv.u;
+        pv = cv.v;
+        if constexpr (REC) {
+          uint32_t code = xs ? (edge ? (pos ? sg.code_x_hi : sg.code_x_lo)
+                                     : (pos ? kMoveXPlus : kMoveXMinus))
+                             : (edge ? (pos ? sg.code_y_hi : sg.code_y_lo)
+                                     : (pos ? kMoveYPlus : kMoveYMinus));
+          code = moves ? code : kMoveStay;
+          code = (alive && ends) ? kMoveStop : code;
+          word |= code << (4 * q);
         }
-        fx = macc<FMA>(ty, mu, fx);
+        alive = live;
       }
-      j = apply_bound(j, g.nx, g.x_left, g.x_right);  // src/lib.rs:112-115
-      i = apply_bound(i, g.ny, g.y_left, g.y_right);
+      words[g] = word;
+      any_alive = __builtin_amdgcn_ballot_w64(alive) != 0;
     }
-    const CellVal<T> c = load_cell(cells, i * g.nx + j);
-    acc = macc<FMA>(kernel[k], c.x, acc);  // src/lib.rs:353-360
-    pu = c.u;
-    pv = c.v;
-    if constexpr (REC) {
-      word |= code << (4 * (s & 7));
-      if ((s & 7) == 7 || s == nsteps - 1) {
-        codes[(w0 + (s >> 3)) * np + p] = word;
-        word = 0;
-      }
-    }
+    if constexpr (REC) codes[(c0 + ch) * np + p] = make_uint4(words[0], words[1], words[2], words[3]);
+    if (!any_alive) break;  // every line of the wave ended
   }
   return acc;
 }
 
-template <typename T, bool FMA, bool BRANCHLESS, bool POL, bool REC>
+template <typename T, bool FMA, bool BRANCHLESS, bool POL, bool REC, bool BUF>
 __global__ __launch_bounds__(256) void lic_trace(const Cell<T>* __restrict__ cells,
                                                  const T* __restrict__ kernel, int32_t klen,
                                                  FieldGeom g, TileMap m, T* __restrict__ out,
-                                                 uint32_t* __restrict__ codes) {
+                                                 uint4* __restrict__ codes) {
   int32_t ty, tx, py, px;
   tile_of_block(blockIdx.x, m, ty, tx);
-  pixel_of_thread(threadIdx.x, py, px);
-  const int32_t i0 = ty * kTile + py, j0 = tx * kTile + px;
+  pixel_of_thread(threadIdx.x, m, py, px);
+  const int32_t i0 = ty * tile_h(m) + py, j0 = tx * tile_w(m) + px;
   if (i0 >= g.ny || j0 >= g.nx) return;
+  const CellGather<T, BUF> cell(cells, static_cast<int64_t>(g.ny) * g.nx);
+  const StepGeom sg = make_step_geom(g, cell.unit);
   const int32_t kmid = klen / 2;
   const int32_t n_fwd = klen - 1 - kmid, n_bwd = kmid;
-  const int64_t np = static_cast<int64_t>(m.n_tiles) * kBlock;
-  const int64_t p = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  const int32_t idx0 = i0 * g.nx + j0;
-  const CellVal<T> c0 = load_cell(cells, idx0);
+  const int64_t np = static_cast<int64_t>(g.ny) * g.nx;  // code slot = pixel index
+  const int64_t p = static_cast<int64_t>(i0) * g.nx + j0;
+  const uint32_t row0 = static_cast<uint32_t>(i0) * sg.nx;
+  const uint32_t jj0 = static_cast<uint32_t>(j0) * cell.unit;
+  const CellVal<T> c0 = cell(row0 + jj0);
   T acc = macc<FMA>(kernel[kmid], c0.x, T(0));  // src/lib.rs:375-383
-  acc = trace_direction<T, FMA, BRANCHLESS, POL, true, REC>(
-      acc, c0.u, c0.v, i0, j0, cells, kernel, kmid, n_fwd, g, codes, np, p, 0);
-  acc = trace_direction<T, FMA, BRANCHLESS, POL, false, REC>(
-      acc, c0.u, c0.v, i0, j0, cells, kernel, kmid, n_bwd, g, codes, np, p,
-      (n_fwd + 7) >> 3);
-  out[idx0] = acc;
+  acc = trace_direction<T, FMA, BRANCHLESS, POL, true, REC, BUF>(
+      acc, c0.u, c0.v, row0, jj0, cell, kernel, kmid, n_fwd, sg, codes, np, p, 0);
+  acc = trace_direction<T, FMA, BRANCHLESS, POL, false, REC, BUF>(
+      acc, c0.u, c0.v, row0, jj0, cell, kernel, kmid, n_bwd, sg, codes, np, p,
+      chunks_of(n_fwd));
+  out[static_cast<int64_t>(i0) * g.nx + j0] = acc;
 }
 
-template <typename T, bool FMA, bool FORWARD>
-__device__ __forceinline__ T replay_direction(T acc, int32_t i, int32_t j,
-                                              const T* __restrict__ x,
+// Replay of one direction from its move codes.  Chunks of 32 steps: the
+// chunk after the current one is loaded before the current one is used, and
+// within a chunk each group of 8 steps decodes its 8 positions through the
+// delta table, issues the 8 texture gathers together and accumulates them
+// in step order.  Codes are valid up to the step where the line ended (the
+// tracing pass wrote the stop code there), so a lane stops reading codes
+// once its line has ended and its later codes are taken as "stay".
+template <typename T, bool FMA, bool FORWARD, bool BUF>
+__device__ __forceinline__ T replay_direction(T acc, uint32_t pos,
+                                              const ValueGather<T, BUF>& x,
                                               const T* __restrict__ kernel, int32_t kmid,
-                                              int32_t nsteps, const FieldGeom& g,
-                                              const uint32_t* __restrict__ codes, int64_t np,
-                                              int64_t p, int32_t w0) {
-  uint32_t word = 0;
-  for (int32_t s = 0; s < nsteps; ++s) {
-    const int32_t k = FORWARD ? kmid + 1 + s : kmid - 1 - s;
-    if ((s & 7) == 0) word = codes[(w0 + (s >> 3)) * np + p];
-    const uint32_t code = (word >> (4 * (s & 7))) & 15u;
-    if (code == kMoveStop) break;
-    j += (code == kMoveXPlus) ? 1 : ((code == kMoveXMinus) ? -1 : 0);
-    i += (code == kMoveYPlus) ? 1 : ((code == kMoveYMinus) ? -1 : 0);
-    j = apply_bound(j, g.nx, g.x_left, g.x_right);
-    i = apply_bound(i, g.ny, g.y_left, g.y_right);
-    acc = macc<FMA>(kernel[k], x[i * g.nx + j], acc);
+                                              int32_t nsteps, const uint32_t* delta,
+                                              const uint4* __restrict__ codes, int64_t np,
+                                              int64_t p, int32_t c0) {
+  const int32_t nchunks = chunks_of(nsteps);
+  bool alive = true;
+  uint4 next = nchunks > 0 ? codes[c0 * np + p] : make_uint4(0u, 0u, 0u, 0u);
+  for (int32_t ch = 0; ch < nchunks; ++ch) {
+    const uint4 cur = next;
+    if (ch + 1 < nchunks && alive) next = codes[(c0 + ch + 1) * np + p];
+    const uint32_t words[4] = {cur.x, cur.y, cur.z, cur.w};
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const int32_t s0 = ch * kChunkSteps + 8 * g;
+      if (s0 >= nsteps) break;  // uniform
+      const uint32_t word = alive ? words[g] : 0u;  // 0 = stay
+      // nibble 0xF marks the step where the line ended
+      const uint32_t stops = word & (word >> 1) & (word >> 2) & (word >> 3) & 0x11111111u;
+      const int32_t valid = alive ? (stops ? (__builtin_ctz(stops) >> 2) : 8) : 0;
+      T kw[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        kw[q] = kernel[FORWARD ? kmid + 1 + s0 + q : kmid - 1 - s0 - q];
+      uint32_t at[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        pos += delta[(word >> (4 * q)) & 15u];
+        at[q] = pos;
+      }
+      T xv[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) xv[q] = x(at[q]);
+      const int32_t lim = nsteps - s0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        if (q >= lim) break;  // uniform: last group only
+        const T next_acc = macc<FMA>(kw[q], xv[q], acc);
+        acc = q < valid ? next_acc : acc;
+      }
+      alive = valid == 8;
+    }
+    if (__builtin_amdgcn_ballot_w64(alive) == 0) break;
   }
   return acc;
 }
 
-template <typename T, bool FMA>
-__global__ __launch_bounds__(256) void lic_replay(const T* __restrict__ x,
+template <typename T, bool FMA, bool BUF>
+__global__ __launch_bounds__(256) void lic_replay(const T* __restrict__ xin,
                                                   const T* __restrict__ kernel, int32_t klen,
                                                   FieldGeom g, TileMap m, T* __restrict__ out,
-                                                  const uint32_t* __restrict__ codes) {
+                                                  const uint4* __restrict__ codes) {
+  const ValueGather<T, BUF> x(xin, static_cast<int64_t>(g.ny) * g.nx);
+  __shared__ uint32_t delta[16];
+  if (threadIdx.x < 16) delta[threadIdx.x] = move_delta(threadIdx.x, g, x.unit);
+  __syncthreads();
   int32_t ty, tx, py, px;
   tile_of_block(blockIdx.x, m, ty, tx);
-  pixel_of_thread(threadIdx.x, py, px);
-  const int32_t i0 = ty * kTile + py, j0 = tx * kTile + px;
+  pixel_of_thread(threadIdx.x, m, py, px);
+  const int32_t i0 = ty * tile_h(m) + py, j0 = tx * tile_w(m) + px;
   if (i0 >= g.ny || j0 >= g.nx) return;
   const int32_t kmid = klen / 2;
   const int32_t n_fwd = klen - 1 - kmid, n_bwd = kmid;
-  const int64_t np = static_cast<int64_t>(m.n_tiles) * kBlock;
-  const int64_t p = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  const int32_t idx0 = i0 * g.nx + j0;
-  T acc = macc<FMA>(kernel[kmid], x[idx0], T(0));
-  acc = replay_direction<T, FMA, true>(acc, i0, j0, x, kernel, kmid, n_fwd, g, codes, np, p, 0);
-  acc = replay_direction<T, FMA, false>(acc, i0, j0, x, kernel, kmid, n_bwd, g, codes, np, p,
-                                        (n_fwd + 7) >> 3);
+  const int64_t np = static_cast<int64_t>(g.ny) * g.nx;  // code slot = pixel index
+  const int64_t idx0 = static_cast<int64_t>(i0) * g.nx + j0;
+  const int64_t p = idx0;
+  const uint32_t pos0 = static_cast<uint32_t>(idx0) * x.unit;
+  T acc = macc<FMA>(kernel[kmid], x(pos0), T(0));
+  acc = replay_direction<T, FMA, true, BUF>(acc, pos0, x, kernel, kmid, n_fwd, delta, codes, np,
+                                            p, 0);
+  acc = replay_direction<T, FMA, false, BUF>(acc, pos0, x, kernel, kmid, n_bwd, delta, codes,
+                                             np, p, chunks_of(n_fwd));
   out[idx0] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// LDS-staged replay.  A block of 1024 threads replays a 32 x 64 pixel tile,
+// two pixels per lane (32 columns apart).  Every step moves one cell, so all
+// lines of the tile stay inside the tile grown by kmid cells on each side
+// (in unwrapped coordinates: a periodic edge is just crossed).  The block
+// stages that box of the texture into LDS once (wrapped modulo the image
+// size), then replays every line from LDS, where a move is +-1 or +-stride
+// whatever the boundary did (a clamp was recorded as "stay").
+//
+// LDS layout: the row stride is 8 (mod 32) elements, which puts the 64
+// lanes of a wave's 8 x 8 pixel square in 64 distinct 4-byte banks
+// (32-bank and 64-bank LDS alike), so an unsheared square reads without
+// bank conflicts.  Moves are decoded two at a time through a 256-entry
+// table indexed by a code byte, holding the two byte-offset deltas as
+// signed 16-bit halves.
+// ---------------------------------------------------------------------------
+constexpr int32_t kLdsTileH = 32;
+constexpr int32_t kLdsTileW = 64;
+constexpr int32_t kLdsThreads = 1024;
+constexpr int32_t kLdsPixelsPerLane = 2;
+
+__host__ __device__ inline int32_t lds_box_h(int32_t klen) { return kLdsTileH + 2 * (klen / 2); }
+__host__ __device__ inline int32_t lds_box_w(int32_t klen) { return kLdsTileW + 2 * (klen / 2); }
+__host__ __device__ inline int32_t lds_box_stride(int32_t klen) {
+  const int32_t w = lds_box_w(klen);
+  return w + ((8 - w) % 32 + 32) % 32;  // == 8 (mod 32)
+}
+
+template <typename T>
+inline size_t lds_replay_bytes(int32_t klen) {
+  return static_cast<size_t>(lds_box_h(klen)) * lds_box_stride(klen) * sizeof(T) +
+         static_cast<size_t>(lds_box_w(klen)) * sizeof(int32_t);
+}
+
+__device__ __forceinline__ int32_t wrap_index(int32_t c, int32_t n) {
+  c %= n;
+  return c < 0 ? c + n : c;
+}
+
+// Byte-offset change in the LDS box for one move code.
+__device__ __forceinline__ int32_t lds_move(uint32_t c, int32_t row_bytes, int32_t elem) {
+  const bool xp = c == kMoveXPlus || c == kMoveXWrapHi;
+  const bool xm = c == kMoveXMinus || c == kMoveXWrapLo;
+  const bool yp = c == kMoveYPlus || c == kMoveYWrapHi;
+  const bool ym = c == kMoveYMinus || c == kMoveYWrapLo;
+  return xp ? elem : xm ? -elem : yp ? row_bytes : ym ? -row_bytes : 0;
+}
+
+template <typename T>
+__device__ __forceinline__ T lds_at(const unsigned char* smem, uint32_t byte_pos) {
+  return *reinterpret_cast<const T*>(smem + byte_pos);
+}
+
+template <typename T, bool FMA, bool FORWARD>
+__device__ __forceinline__ void replay_directions_lds(
+    T (&acc)[kLdsPixelsPerLane], const uint32_t (&pos0)[kLdsPixelsPerLane],
+    const bool (&inside)[kLdsPixelsPerLane], const unsigned char* box,
+    const T* __restrict__ kernel, int32_t kmid, int32_t nsteps, const uint32_t* pair_delta,
+    const uint4* __restrict__ codes, int64_t np, const int64_t (&p)[kLdsPixelsPerLane],
+    int32_t c0) {
+  constexpr int E = kLdsPixelsPerLane;
+  const int32_t nchunks = chunks_of(nsteps);
+  bool alive[E];
+  uint32_t pos[E];
+  uint4 next[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    alive[e] = inside[e];
+    pos[e] = pos0[e];
+    next[e] = (nchunks > 0 && alive[e]) ? codes[c0 * np + p[e]] : make_uint4(0u, 0u, 0u, 0u);
+  }
+  for (int32_t ch = 0; ch < nchunks; ++ch) {
+    uint32_t words[E][4];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      words[e][0] = next[e].x;
+      words[e][1] = next[e].y;
+      words[e][2] = next[e].z;
+      words[e][3] = next[e].w;
+      if (ch + 1 < nchunks && alive[e]) next[e] = codes[(c0 + ch + 1) * np + p[e]];
+    }
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const int32_t s0 = ch * kChunkSteps + 8 * g;
+      if (s0 >= nsteps) break;  // uniform
+      T kw[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        kw[q] = kernel[FORWARD ? kmid + 1 + s0 + q : kmid - 1 - s0 - q];
+      const int32_t lim = nsteps - s0;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const uint32_t word = alive[e] ? words[e][g] : 0u;  // 0 = stay
+        const uint32_t stops = word & (word >> 1) & (word >> 2) & (word >> 3) & 0x11111111u;
+        const int32_t valid = alive[e] ? (stops ? (__builtin_ctz(stops) >> 2) : 8) : 0;
+        T xv[8];
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+          const uint32_t d2 = pair_delta[(word >> (8 * h)) & 255u];
+          pos[e] += static_cast<uint32_t>(static_cast<int32_t>(static_cast<int16_t>(d2 & 0xFFFFu)));
+          xv[2 * h] = lds_at<T>(box, pos[e]);
+          pos[e] += static_cast<uint32_t>(static_cast<int32_t>(d2) >> 16);
+          xv[2 * h + 1] = lds_at<T>(box, pos[e]);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          if (q >= lim) break;  // uniform: last group only
+          const T next_acc = macc<FMA>(kw[q], xv[q], acc[e]);
+          acc[e] = q < valid ? next_acc : acc[e];
+        }
+        alive[e] = valid == 8;
+      }
+    }
+    bool any = false;
+#pragma unroll
+    for (int e = 0; e < E; ++e) any = any || alive[e];
+    if (__builtin_amdgcn_ballot_w64(any) == 0) break;
+  }
+}
+
+template <typename T, bool FMA>
+__global__ __launch_bounds__(kLdsThreads) void lic_replay_lds(const T* __restrict__ x,
+                                                              const T* __restrict__ kernel,
+                                                              int32_t klen, FieldGeom g,
+                                                              TileMap m, T* __restrict__ out,
+                                                              const uint4* __restrict__ codes) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int E = kLdsPixelsPerLane;
+  const int32_t kmid = klen / 2;
+  const int32_t bh = lds_box_h(klen), bw = lds_box_w(klen), stride = lds_box_stride(klen);
+  int32_t* col_map = reinterpret_cast<int32_t*>(smem + sizeof(T) * bh * stride);
+  __shared__ uint32_t pair_delta[256];
+
+  int32_t ty, tx;
+  tile_of_block(blockIdx.x, m, ty, tx);
+  const int32_t i_box0 = ty * kLdsTileH - kmid, j_box0 = tx * kLdsTileW - kmid;
+  const int32_t tid = threadIdx.x;
+  const int32_t row_bytes = stride * static_cast<int32_t>(sizeof(T));
+  if (tid < 256) {
+    const int32_t lo = lds_move(tid & 15, row_bytes, sizeof(T));
+    const int32_t hi = lds_move(tid >> 4, row_bytes, sizeof(T));
+    pair_delta[tid] = (static_cast<uint32_t>(hi) << 16) | (static_cast<uint32_t>(lo) & 0xFFFFu);
+  }
+  for (int32_t b = tid; b < bw; b += kLdsThreads) col_map[b] = wrap_index(j_box0 + b, g.nx);
+  __syncthreads();
+  // stage the box: wave w copies rows w, w + 16, ... (coalesced along x)
+  T* box_t = reinterpret_cast<T*>(smem);
+  const int32_t wave = tid >> 6, lane = tid & 63;
+  for (int32_t a = wave; a < bh; a += kLdsThreads / 64) {
+    const int64_t src_row = static_cast<int64_t>(wrap_index(i_box0 + a, g.ny)) * g.nx;
+    for (int32_t b = lane; b < bw; b += 64) box_t[a * stride + b] = x[src_row + col_map[b]];
+  }
+  __syncthreads();
+
+  // wave w covers the 8 x 8 squares at (8 * (w / 4), 8 * (w % 4)) and 32 columns to the right
+  const int32_t py = ((wave >> 2) << 3) | (lane >> 3);
+  const int32_t px0 = ((wave & 3) << 3) | (lane & 7);
+  const int32_t n_fwd = klen - 1 - kmid, n_bwd = kmid;
+  const int64_t np = static_cast<int64_t>(g.ny) * g.nx;
+  T acc[E];
+  uint32_t pos0[E];
+  bool inside[E];
+  int64_t p[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int32_t px = px0 + 32 * e;
+    const int32_t i0 = ty * kLdsTileH + py, j0 = tx * kLdsTileW + px;
+    inside[e] = i0 < g.ny && j0 < g.nx;
+    p[e] = inside[e] ? static_cast<int64_t>(i0) * g.nx + j0 : 0;
+    pos0[e] = static_cast<uint32_t>(((kmid + py) * stride + (kmid + px)) * sizeof(T));
+    acc[e] = macc<FMA>(kernel[kmid], lds_at<T>(smem, pos0[e]), T(0));
+  }
+  replay_directions_lds<T, FMA, true>(acc, pos0, inside, smem, kernel, kmid, n_fwd, pair_delta,
+                                      codes, np, p, 0);
+  replay_directions_lds<T, FMA, false>(acc, pos0, inside, smem, kernel, kmid, n_bwd, pair_delta,
+                                       codes, np, p, chunks_of(n_fwd));
+#pragma unroll
+  for (int e = 0; e < E; ++e)
+    if (inside[e]) out[p[e]] = acc[e];
 }
 
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 
-TileMap make_tile_map(int32_t ny, int32_t nx, int32_t band) {
+TileMap make_tile_map(int32_t ny, int32_t nx, int32_t band, int32_t wave_w_log2) {
   TileMap m;
-  m.tiles_x = (nx + kTile - 1) / kTile;
-  m.tiles_y = (ny + kTile - 1) / kTile;
+  m.wave_w_log2 = wave_w_log2 < 3 ? 3 : (wave_w_log2 > 5 ? 5 : wave_w_log2);
+  const int32_t tw = tile_w(m), th = tile_h(m);
+  m.tiles_x = (nx + tw - 1) / tw;
+  m.tiles_y = (ny + th - 1) / th;
   m.n_tiles = m.tiles_x * m.tiles_y;
   m.per_xcd = m.n_tiles / kXcds;
   m.rem_xcd = m.n_tiles % kXcds;
@@ -215,10 +486,10 @@ TileMap make_tile_map(int32_t ny, int32_t nx, int32_t band) {
   return m;
 }
 
-int32_t code_words(int32_t klen) {
+int32_t code_chunks(int32_t klen) {
   const int32_t kmid = klen / 2;
   const int32_t n_fwd = klen - 1 - kmid, n_bwd = kmid;
-  return ((n_fwd + 7) >> 3) + ((n_bwd + 7) >> 3);
+  return chunks_of(n_fwd) + chunks_of(n_bwd);
 }
 
 template <typename T>
@@ -228,6 +499,18 @@ hipError_t launch_pack(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64
   dim3 grid((nx + 63) / 64, (ny + 3) / 4);
   hipLaunchKernelGGL(lic_pack<T>, grid, dim3(256), 0, stream, tex, tsy, tsx, u, usy, usx, v,
                      vsy, vsx, ny, nx, cells);
+  return hipGetLastError();
+}
+
+int32_t kernel_pad() { return kKernelPad; }
+
+template <typename T>
+hipError_t launch_pad_kernel(const T* kernel, int32_t klen, T* dst, hipStream_t stream) {
+  const int32_t total = klen + 2 * kKernelPad;
+  int32_t blocks = (total + 255) / 256;
+  if (blocks > 1024) blocks = 1024;
+  hipLaunchKernelGGL(lic_pad_kernel<T>, dim3(blocks), dim3(256), 0, stream, kernel, klen,
+                     kKernelPad, dst);
   return hipGetLastError();
 }
 
@@ -246,19 +529,28 @@ hipError_t launch_copy2d(const T* src, int64_t sy, int64_t sx, int32_t ny, int32
   return hipGetLastError();
 }
 
+// Buffer addressing covers arrays of fewer than 2^32 bytes.
+inline bool fits_buffer(int64_t bytes) { return bytes > 0 && bytes <= 0xFFFFFFFFll; }
+
 template <typename T, bool FMA, bool BL, bool POL, bool REC>
 static hipError_t trace_one(const Cell<T>* cells, const T* kernel, int32_t klen,
-                            const FieldGeom& g, const TileMap& m, T* out, uint32_t* codes,
+                            const FieldGeom& g, const TileMap& m, T* out, uint4* codes,
                             hipStream_t stream) {
-  hipLaunchKernelGGL((lic_trace<T, FMA, BL, POL, REC>), dim3(m.n_tiles), dim3(kBlock), 0,
-                     stream, cells, kernel, klen, g, m, out, codes);
+  const int64_t bytes = static_cast<int64_t>(g.ny) * g.nx * sizeof(Cell<T>);
+  if (fits_buffer(bytes)) {
+    hipLaunchKernelGGL((lic_trace<T, FMA, BL, POL, REC, true>), dim3(m.n_tiles), dim3(kBlock),
+                       0, stream, cells, kernel, klen, g, m, out, codes);
+  } else {
+    hipLaunchKernelGGL((lic_trace<T, FMA, BL, POL, REC, false>), dim3(m.n_tiles), dim3(kBlock),
+                       0, stream, cells, kernel, klen, g, m, out, codes);
+  }
   return hipGetLastError();
 }
 
 template <typename T, bool FMA, bool BL>
 static hipError_t trace_mode(const Cell<T>* cells, const T* kernel, int32_t klen,
                              const FieldGeom& g, const TileMap& m, int uv_mode, T* out,
-                             uint32_t* codes, hipStream_t stream) {
+                             uint4* codes, hipStream_t stream) {
   const bool rec = codes != nullptr;
   if (uv_mode == 1) {
     return rec ? trace_one<T, FMA, BL, true, true>(cells, kernel, klen, g, m, out, codes, stream)
@@ -271,7 +563,7 @@ static hipError_t trace_mode(const Cell<T>* cells, const T* kernel, int32_t klen
 
 template <typename T>
 hipError_t launch_trace(const Cell<T>* cells, const T* kernel, int32_t klen, const FieldGeom& g,
-                        const TileMap& m, int uv_mode, int profile, T* out, uint32_t* codes,
+                        const TileMap& m, int uv_mode, int profile, T* out, uint4* codes,
                         hipStream_t stream) {
   switch (profile & 3) {
     case 0:
@@ -287,14 +579,66 @@ hipError_t launch_trace(const Cell<T>* cells, const T* kernel, int32_t klen, con
 
 template <typename T>
 hipError_t launch_replay(const T* x, const T* kernel, int32_t klen, const FieldGeom& g,
-                         const TileMap& m, int profile, T* out, const uint32_t* codes,
+                         const TileMap& m, int profile, T* out, const uint4* codes,
                          hipStream_t stream) {
+  const bool buf = fits_buffer(static_cast<int64_t>(g.ny) * g.nx * sizeof(T));
   if (profile & 1) {
-    hipLaunchKernelGGL((lic_replay<T, true>), dim3(m.n_tiles), dim3(kBlock), 0, stream, x,
-                       kernel, klen, g, m, out, codes);
+    if (buf) {
+      hipLaunchKernelGGL((lic_replay<T, true, true>), dim3(m.n_tiles), dim3(kBlock), 0, stream,
+                         x, kernel, klen, g, m, out, codes);
+    } else {
+      hipLaunchKernelGGL((lic_replay<T, true, false>), dim3(m.n_tiles), dim3(kBlock), 0, stream,
+                         x, kernel, klen, g, m, out, codes);
+    }
   } else {
-    hipLaunchKernelGGL((lic_replay<T, false>), dim3(m.n_tiles), dim3(kBlock), 0, stream, x,
-                       kernel, klen, g, m, out, codes);
+    if (buf) {
+      hipLaunchKernelGGL((lic_replay<T, false, true>), dim3(m.n_tiles), dim3(kBlock), 0, stream,
+                         x, kernel, klen, g, m, out, codes);
+    } else {
+      hipLaunchKernelGGL((lic_replay<T, false, false>), dim3(m.n_tiles), dim3(kBlock), 0,
+                         stream, x, kernel, klen, g, m, out, codes);
+    }
+  }
+  return hipGetLastError();
+}
+
+int32_t lds_replay_limit_bytes() {
+  int dev = 0, limit = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+    return 0;
+  return limit;
+}
+
+template <typename T>
+bool lds_replay_fits(int32_t klen) {
+  return lds_replay_bytes<T>(klen) + 64 <= static_cast<size_t>(lds_replay_limit_bytes());
+}
+
+template <typename T>
+hipError_t launch_replay_lds(const T* x, const T* kernel, int32_t klen, const FieldGeom& g,
+                             int profile, T* out, const uint4* codes, hipStream_t stream) {
+  TileMap m;
+  m.wave_w_log2 = 4;  // unused by this kernel except through tile_of_block
+  m.tiles_x = (g.nx + kLdsTileW - 1) / kLdsTileW;
+  m.tiles_y = (g.ny + kLdsTileH - 1) / kLdsTileH;
+  m.n_tiles = m.tiles_x * m.tiles_y;
+  m.per_xcd = m.n_tiles / kXcds;
+  m.rem_xcd = m.n_tiles % kXcds;
+  m.band = 4;
+  const size_t bytes = lds_replay_bytes<T>(klen);
+  if (profile & 1) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&lic_replay_lds<T, true>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    (void)hipGetLastError();  // the attribute is advisory on this target
+    hipLaunchKernelGGL((lic_replay_lds<T, true>), dim3(m.n_tiles), dim3(kLdsThreads), bytes,
+                       stream, x, kernel, klen, g, m, out, codes);
+  } else {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&lic_replay_lds<T, false>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    (void)hipGetLastError();  // the attribute is advisory on this target
+    hipLaunchKernelGGL((lic_replay_lds<T, false>), dim3(m.n_tiles), dim3(kLdsThreads), bytes,
+                       stream, x, kernel, klen, g, m, out, codes);
   }
   return hipGetLastError();
 }
@@ -304,12 +648,16 @@ hipError_t launch_replay(const T* x, const T* kernel, int32_t klen, const FieldG
                                      const T*, int64_t, int64_t, int32_t, int32_t, Cell<T>*,   \
                                      hipStream_t);                                             \
   template hipError_t launch_repack_texture<T>(const T*, int32_t, Cell<T>*, hipStream_t);      \
+  template hipError_t launch_pad_kernel<T>(const T*, int32_t, T*, hipStream_t);                \
   template hipError_t launch_copy2d<T>(const T*, int64_t, int64_t, int32_t, int32_t, T*,       \
                                        hipStream_t);                                           \
   template hipError_t launch_trace<T>(const Cell<T>*, const T*, int32_t, const FieldGeom&,     \
-                                      const TileMap&, int, int, T*, uint32_t*, hipStream_t);   \
+                                      const TileMap&, int, int, T*, uint4*, hipStream_t);   \
   template hipError_t launch_replay<T>(const T*, const T*, int32_t, const FieldGeom&,          \
-                                       const TileMap&, int, T*, const uint32_t*, hipStream_t);
+                                       const TileMap&, int, T*, const uint4*, hipStream_t);    \
+  template bool lds_replay_fits<T>(int32_t);                                                   \
+  template hipError_t launch_replay_lds<T>(const T*, const T*, int32_t, const FieldGeom&, int, \
+                                           T*, const uint4*, hipStream_t);
 
 RLIC_INSTANTIATE(float)
 RLIC_INSTANTIATE(double)

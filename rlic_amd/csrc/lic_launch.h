@@ -8,16 +8,25 @@
 
 namespace rlic {
 
-TileMap make_tile_map(int32_t ny, int32_t nx, int32_t band);
+// wave_w_log2: log2 of the wave tile width (3: 8x8, 4: 16x4, 5: 32x2 pixels).
+TileMap make_tile_map(int32_t ny, int32_t nx, int32_t band, int32_t wave_w_log2);
 
-// Number of 32-bit move-code words recorded per pixel for a kernel of
-// length klen (forward and backward directions are word-aligned apart).
-int32_t code_words(int32_t klen);
+// Number of 16-byte move-code chunks (32 steps each) recorded per pixel
+// slot for a kernel of length klen (forward chunks, then backward chunks).
+int32_t code_chunks(int32_t klen);
 
 template <typename T>
 hipError_t launch_pack(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t usy,
                        int64_t usx, const T* v, int64_t vsy, int64_t vsx, int32_t ny,
                        int32_t nx, Cell<T>* cells, hipStream_t stream);
+
+// Zeros on each side of the padded kernel-weight copy the streamline
+// kernels read (they load taps in groups of 8 without bounds checks).
+int32_t kernel_pad();
+
+// dst (klen + 2 * kernel_pad() elements) = [zeros][kernel][zeros]
+template <typename T>
+hipError_t launch_pad_kernel(const T* kernel, int32_t klen, T* dst, hipStream_t stream);
 
 template <typename T>
 hipError_t launch_repack_texture(const T* x, int32_t n, Cell<T>* cells, hipStream_t stream);
@@ -26,16 +35,26 @@ template <typename T>
 hipError_t launch_copy2d(const T* src, int64_t sy, int64_t sx, int32_t ny, int32_t nx, T* dst,
                          hipStream_t stream);
 
+// `kernel` points at the first weight of a padded copy (launch_pad_kernel).
 // codes == nullptr: no recording.  Otherwise codes holds
-// code_words(klen) * n_tiles * 256 words.
+// code_chunks(klen) * n_tiles * 256 chunks.
 template <typename T>
 hipError_t launch_trace(const Cell<T>* cells, const T* kernel, int32_t klen, const FieldGeom& g,
-                        const TileMap& m, int uv_mode, int profile, T* out, uint32_t* codes,
+                        const TileMap& m, int uv_mode, int profile, T* out, uint4* codes,
                         hipStream_t stream);
 
 template <typename T>
 hipError_t launch_replay(const T* x, const T* kernel, int32_t klen, const FieldGeom& g,
-                         const TileMap& m, int profile, T* out, const uint32_t* codes,
+                         const TileMap& m, int profile, T* out, const uint4* codes,
                          hipStream_t stream);
+
+// LDS-staged replay (32 x 32 tiles): usable when the tile plus its kmid
+// halo fits the device's per-block LDS.
+template <typename T>
+bool lds_replay_fits(int32_t klen);
+
+template <typename T>
+hipError_t launch_replay_lds(const T* x, const T* kernel, int32_t klen, const FieldGeom& g,
+                             int profile, T* out, const uint4* codes, hipStream_t stream);
 
 }  // namespace rlic

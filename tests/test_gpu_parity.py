@@ -227,3 +227,61 @@ def test_torch_device_tensors(gpu_device):
                              np.ascontiguousarray(u.T), kernel=k,
                              boundaries={"x": "closed", "y": "periodic"})
         _assert_same(got_t.cpu().numpy(), exp_t, "torch transposed")
+
+
+@pytest.mark.parametrize("wave_w_log2", ["3", "4", "5"])
+@pytest.mark.parametrize("band", ["1", "16"])
+def test_launch_shapes_do_not_change_results(wave_w_log2, band, monkeypatch):
+    """Wave tile shape and tile ordering only move work between lanes."""
+    monkeypatch.setenv("RLIC_WAVE_W_LOG2", wave_w_log2)
+    monkeypatch.setenv("RLIC_TILE_BAND", band)
+    rng = np.random.default_rng(int(wave_w_log2) * 10 + int(band))
+    tex, u, v = _fields(rng, (75, 131), np.float32, "holes")
+    k = rng.random(25).astype(np.float32)
+    for b in ("closed", {"x": "periodic", "y": ("closed", "closed")}):
+        exp = cpu.convolve(tex, u, v, kernel=k, uv_mode="polarization", boundaries=b,
+                           iterations=3)
+        got = rlic_amd.convolve(tex, u, v, kernel=k, uv_mode="polarization", boundaries=b,
+                                iterations=3)
+        _assert_same(got, exp, f"wave 2^{wave_w_log2} band {band} {b}")
+
+
+def test_library_and_torch_share_one_hip_runtime():
+    """Host-array calls before `import torch` must leave torch's device usable."""
+    import subprocess
+    import sys
+    from pathlib import Path
+
+    root = Path(__file__).resolve().parents[1]
+    code = (
+        "import sys; sys.path.insert(0, %r)\n"
+        "import numpy as np, rlic_amd\n"
+        "a = np.random.default_rng(0).random((16, 16))\n"
+        "rlic_amd.convolve(a, a, a, kernel=np.ones(5))\n"
+        "import torch\n"
+        "assert torch.cuda.is_available(), 'torch lost the device'\n"
+        "t = torch.from_numpy(a).cuda()\n"
+        "out = rlic_amd.convolve(t, t, t, kernel=torch.ones(5, dtype=t.dtype, device=t.device))\n"
+        "assert np.array_equal(out.cpu().numpy(), rlic_amd.convolve(a, a, a, kernel=np.ones(5)))\n"
+        "print('ok')\n" % str(root)
+    )
+    proc = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                          timeout=240)
+    assert proc.returncode == 0 and proc.stdout.strip().endswith("ok"), proc.stdout + proc.stderr
+
+
+@pytest.mark.parametrize("lds", ["0", "1"])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_replay_paths_match(lds, dtype, monkeypatch):
+    """Replay from LDS-staged tiles and from global gathers give the same bits,
+    including lines that wrap across periodic edges of images smaller than
+    the staged box."""
+    monkeypatch.setenv("RLIC_REPLAY_LDS", lds)
+    rng = np.random.default_rng(21 + int(lds))
+    for shape, klen in (((70, 90), 33), ((20, 24), 65), ((33, 47), 64)):
+        tex, u, v = _fields(rng, shape, dtype, "holes")
+        k = rng.random(klen).astype(dtype)
+        for b in ("periodic", {"x": "periodic", "y": "closed"}, "closed"):
+            exp = cpu.convolve(tex, u, v, kernel=k, boundaries=b, iterations=3)
+            got = rlic_amd.convolve(tex, u, v, kernel=k, boundaries=b, iterations=3)
+            _assert_same(got, exp, f"lds={lds} {shape} L={klen} {b}")
