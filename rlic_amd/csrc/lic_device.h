@@ -126,6 +126,16 @@ struct ValueGather {
   }
 };
 
+// A copy of a (wave-uniform) value that the compiler must treat as
+// lane-varying: loop-invariant select operands kept this way live in VGPRs,
+// so each select is one v_cndmask instead of a scalar-to-vector move (a
+// VALU instruction reads at most one scalar register) or a branch.
+__device__ __forceinline__ uint32_t lane_copy(uint32_t x) {
+  uint32_t r;
+  asm("v_mov_b32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float abs_t(float a) { return __builtin_fabsf(a); }
@@ -173,31 +183,27 @@ struct FieldGeom {
 
 // Per-field constants of the incremental step rule.  A position is kept as
 // (row_off, j) with row_off = i * nx, so the cell index is row_off + j and a
-// step changes one of the two by +-1 / +-nx, or jumps to the target of the
-// edge it crossed (src/lib.rs:83-95: closed clamps to the edge cell,
-// periodic wraps to the opposite edge).
+// step changes one of the two by +-1 / +-nx, or, stepping off an edge, the
+// boundary rule applies (src/lib.rs:83-95): a closed edge keeps the edge
+// cell, a periodic edge wraps to the opposite edge cell.  At an edge the
+// coordinate is 0 or its maximum m, so the wrapped value is m - c.
 struct StepGeom {
-  uint32_t nx, nxm1, last_row;        // last_row = (ny - 1) * nx
-  uint32_t x_lo_target, x_hi_target;  // j after stepping off the left / right edge
-  uint32_t y_lo_target, y_hi_target;  // row_off after stepping off the top / bottom edge
-  uint32_t code_x_lo, code_x_hi;     // move code recorded for such a step
-  uint32_t code_y_lo, code_y_hi;
+  uint32_t nx, nxm1, last_row;  // row step, last column, last row offset
+  bool x_lo_per, x_hi_per, y_lo_per, y_hi_per;  // periodic sides
 };
 
-// Move codes recorded by the tracing pass (4 bits per step, 8 per word).
-// A code names the cell-index change of one step with the boundary rule
-// already applied, so replay is a table lookup: a clamped step is a stay.
+// Move codes recorded by the tracing pass (4 bits per step, 8 per word):
+// the direction, plus kMoveEdge when the step left the image (the boundary
+// rule of that side then applies), kMoveStay for no move and kMoveStop where
+// the streamline ended on a NaN vector (src/lib.rs:336-338).
 enum : uint32_t {
   kMoveStay = 0,
   kMoveXPlus = 1,
   kMoveXMinus = 2,
   kMoveYPlus = 3,
   kMoveYMinus = 4,
-  kMoveXWrapHi = 5,  // j: nx-1 -> 0
-  kMoveXWrapLo = 6,  // j: 0 -> nx-1
-  kMoveYWrapHi = 7,  // i: ny-1 -> 0
-  kMoveYWrapLo = 8,  // i: 0 -> ny-1
-  kMoveStop = 15     // streamline ended on a NaN vector (src/lib.rs:336-338)
+  kMoveEdge = 8,
+  kMoveStop = 15
 };
 
 // Positions are in `unit`s: 1 for element indices, the element size for
@@ -207,32 +213,38 @@ __host__ __device__ inline StepGeom make_step_geom(const FieldGeom& g, uint32_t 
   s.nx = static_cast<uint32_t>(g.nx) * unit;
   s.nxm1 = static_cast<uint32_t>(g.nx - 1) * unit;
   s.last_row = static_cast<uint32_t>(g.ny - 1) * static_cast<uint32_t>(g.nx) * unit;
-  s.x_lo_target = g.x_left ? s.nxm1 : 0;
-  s.x_hi_target = g.x_right ? 0 : s.nxm1;
-  s.y_lo_target = g.y_left ? s.last_row : 0;
-  s.y_hi_target = g.y_right ? 0 : s.last_row;
-  s.code_x_lo = g.x_left ? kMoveXWrapLo : kMoveStay;
-  s.code_x_hi = g.x_right ? kMoveXWrapHi : kMoveStay;
-  s.code_y_lo = g.y_left ? kMoveYWrapLo : kMoveStay;
-  s.code_y_hi = g.y_right ? kMoveYWrapHi : kMoveStay;
+  s.x_lo_per = g.x_left != 0;
+  s.x_hi_per = g.x_right != 0;
+  s.y_lo_per = g.y_left != 0;
+  s.y_hi_per = g.y_right != 0;
   return s;
 }
 
-// Position change of each move code in `unit`s (replay's lookup table).
-// Deltas are added modulo 2^32 (positions are uint32_t).
-__host__ __device__ inline uint32_t move_delta(uint32_t code, const FieldGeom& g, uint32_t unit) {
-  const uint32_t nx = static_cast<uint32_t>(g.nx), ny = static_cast<uint32_t>(g.ny);
+// Position change of each move code in `unit`s, added modulo 2^32
+// (positions are uint32_t).  `row` is the row length and `span_x`, `span_y`
+// the distance between opposite edge cells, all in units; a periodic edge
+// step jumps across the image, a closed one stays.
+__host__ __device__ inline uint32_t move_delta_units(uint32_t code, const FieldGeom& g,
+                                                     uint32_t unit, uint32_t row,
+                                                     uint32_t span_x, uint32_t span_y) {
   switch (code) {
     case kMoveXPlus: return unit;
     case kMoveXMinus: return 0u - unit;
-    case kMoveYPlus: return nx * unit;
-    case kMoveYMinus: return 0u - nx * unit;
-    case kMoveXWrapHi: return 0u - (nx - 1) * unit;
-    case kMoveXWrapLo: return (nx - 1) * unit;
-    case kMoveYWrapHi: return 0u - (ny - 1) * nx * unit;
-    case kMoveYWrapLo: return (ny - 1) * nx * unit;
+    case kMoveYPlus: return row;
+    case kMoveYMinus: return 0u - row;
+    case kMoveXPlus | kMoveEdge: return g.x_right ? 0u - span_x : 0u;
+    case kMoveXMinus | kMoveEdge: return g.x_left ? span_x : 0u;
+    case kMoveYPlus | kMoveEdge: return g.y_right ? 0u - span_y : 0u;
+    case kMoveYMinus | kMoveEdge: return g.y_left ? span_y : 0u;
     default: return 0u;
   }
+}
+
+// Replay through the image itself: wraps jump across it.
+__host__ __device__ inline uint32_t move_delta(uint32_t code, const FieldGeom& g, uint32_t unit) {
+  const uint32_t row = static_cast<uint32_t>(g.nx) * unit;
+  return move_delta_units(code, g, unit, row, static_cast<uint32_t>(g.nx - 1) * unit,
+                          static_cast<uint32_t>(g.ny - 1) * row);
 }
 
 // Block -> pixel tile.  A block is 4 wave64s in a 2x2 arrangement; each wave

@@ -98,11 +98,20 @@ __device__ __forceinline__ T trace_direction(T acc, T pu, T pv, uint32_t row, ui
   T last_u = T(0), last_v = T(0);  // src/lib.rs:320-323
   bool alive = true;
   const uint32_t unit = cell.unit;
+  // select operands: edge positions, edge targets (src/lib.rs:83-95: a
+  // closed side keeps the edge cell, a periodic side wraps to the opposite
+  // edge cell) and row steps, held in VGPRs across the loop
+  const uint32_t nxm1 = lane_copy(sg.nxm1), last_row = lane_copy(sg.last_row);
+  const uint32_t x_hi = lane_copy(sg.x_hi_per ? 0u : sg.nxm1);
+  const uint32_t x_lo = lane_copy(sg.x_lo_per ? sg.nxm1 : 0u);
+  const uint32_t y_hi = lane_copy(sg.y_hi_per ? 0u : sg.last_row);
+  const uint32_t y_lo = lane_copy(sg.y_lo_per ? sg.last_row : 0u);
+  const uint32_t row_up = lane_copy(sg.nx), row_down = lane_copy(0u - sg.nx);
   const int32_t nchunks = chunks_of(nsteps);
   for (int32_t ch = 0; ch < nchunks; ++ch) {
     uint32_t words[4] = {0u, 0u, 0u, 0u};
     bool any_alive = true;
-#pragma unroll
+#pragma unroll 1
     for (int g = 0; g < 4; ++g) {
       const int32_t s0 = ch * kChunkSteps + 8 * g;
       if (s0 >= nsteps || !any_alive) break;  // uniform
@@ -134,34 +143,31 @@ __device__ __forceinline__ T trace_direction(T acc, T pu, T pv, uint32_t row, ui
         const bool xs = tx < ty;
         // no move when both components are +-0 (src/lib.rs:242-244)
         const bool moves = live && (mu != T(0) || mv != T(0));
-        // update_state (src/lib.rs:208-233) on the crossed axis
-        const T vpar = xs ? mu : mv;
-        const bool pos = vpar >= T(0);
-        const T frac_par = pos ? T(0) : T(1);
-        const T frac_orth = macc<FMA>(xs ? tx : ty, xs ? mv : mu, xs ? fy : fx);
-        fx = moves ? (xs ? frac_par : frac_orth) : fx;
-        fy = moves ? (xs ? frac_orth : frac_par) : fy;
-        // coordinate step with the boundary rule (src/lib.rs:83-95, 112-115)
-        const uint32_t c = xs ? j : row;
-        const uint32_t edge_at = xs ? (pos ? sg.nxm1 : 0u) : (pos ? sg.last_row : 0u);
-        const uint32_t stride = xs ? unit : sg.nx;
-        const uint32_t target = xs ? (pos ? sg.x_hi_target : sg.x_lo_target)
-                                   : (pos ? sg.y_hi_target : sg.y_lo_target);
-        const bool edge = c == edge_at;
-        const uint32_t moved = edge ? target : (pos ? c + stride : c - stride);
-        j = (moves && xs) ? moved : j;
-        row = (moves && !xs) ? moved : row;
+        // update_state (src/lib.rs:208-233) for either axis, then keep one
+        const bool pos_x = mu >= T(0);
+        const bool pos_y = mv >= T(0);
+        const T fy_after_x = macc<FMA>(tx, mv, fy);
+        const T fx_after_y = macc<FMA>(ty, mu, fx);
+        const T fx_new = xs ? (pos_x ? T(0) : T(1)) : fx_after_y;
+        const T fy_new = xs ? fy_after_x : (pos_y ? T(0) : T(1));
+        fx = moves ? fx_new : fx;
+        fy = moves ? fy_new : fy;
+        // the crossed coordinate with the boundary rule (src/lib.rs:83-95)
+        const bool edge_x = j == (pos_x ? nxm1 : 0u);
+        const uint32_t j_to = edge_x ? (pos_x ? x_hi : x_lo) : j + (pos_x ? unit : 0u - unit);
+        const bool edge_y = row == (pos_y ? last_row : 0u);
+        const uint32_t row_to = edge_y ? (pos_y ? y_hi : y_lo) : row + (pos_y ? row_up : row_down);
+        j = (moves && xs) ? j_to : j;
+        row = (moves && !xs) ? row_to : row;
         const CellVal<T> cv = cell(row + j);
         const T next_acc = macc<FMA>(kw[q], cv.x, acc);  // src/lib.rs:353-360
         acc = live ? next_acc : acc;
         pu = cv.u;
         pv = cv.v;
         if constexpr (REC) {
-          uint32_t code = xs ? (edge ? (pos ? sg.code_x_hi : sg.code_x_lo)
-                                     : (pos ? kMoveXPlus : kMoveXMinus))
-                             : (edge ? (pos ? sg.code_y_hi : sg.code_y_lo)
-                                     : (pos ? kMoveYPlus : kMoveYMinus));
-          code = moves ? code : kMoveStay;
+          const uint32_t code_x = (pos_x ? kMoveXPlus : kMoveXMinus) | (edge_x ? kMoveEdge : 0u);
+          const uint32_t code_y = (pos_y ? kMoveYPlus : kMoveYMinus) | (edge_y ? kMoveEdge : 0u);
+          uint32_t code = moves ? (xs ? code_x : code_y) : kMoveStay;
           code = (alive && ends) ? kMoveStop : code;
           word |= code << (4 * q);
         }
@@ -289,13 +295,14 @@ __global__ __launch_bounds__(256) void lic_replay(const T* __restrict__ xin,
 }
 
 // ---------------------------------------------------------------------------
-// LDS-staged replay.  A block of 1024 threads replays a 32 x 64 pixel tile,
-// two pixels per lane (32 columns apart).  Every step moves one cell, so all
-// lines of the tile stay inside the tile grown by kmid cells on each side
-// (in unwrapped coordinates: a periodic edge is just crossed).  The block
-// stages that box of the texture into LDS once (wrapped modulo the image
-// size), then replays every line from LDS, where a move is +-1 or +-stride
-// whatever the boundary did (a clamp was recorded as "stay").
+// LDS-staged replay.  A block of 1024 threads replays a TH x TW pixel tile
+// (TH, TW in {32, 64}), E = TH * TW / 1024 pixels per lane.  Every step
+// moves one cell, so all lines of the tile stay inside the tile grown by
+// kmid cells on each side (in unwrapped coordinates: a periodic edge is
+// just crossed).  The block stages that box of the texture into LDS once
+// (wrapped modulo the image size), then replays every line from LDS, where
+// a move is +-1 or +-stride whatever the boundary did (a clamp was recorded
+// as "stay").
 //
 // LDS layout: the row stride is 8 (mod 32) elements, which puts the 64
 // lanes of a wave's 8 x 8 pixel square in 64 distinct 4-byte banks
@@ -304,22 +311,18 @@ __global__ __launch_bounds__(256) void lic_replay(const T* __restrict__ xin,
 // table indexed by a code byte, holding the two byte-offset deltas as
 // signed 16-bit halves.
 // ---------------------------------------------------------------------------
-constexpr int32_t kLdsTileH = 32;
-constexpr int32_t kLdsTileW = 64;
 constexpr int32_t kLdsThreads = 1024;
-constexpr int32_t kLdsPixelsPerLane = 2;
 
-__host__ __device__ inline int32_t lds_box_h(int32_t klen) { return kLdsTileH + 2 * (klen / 2); }
-__host__ __device__ inline int32_t lds_box_w(int32_t klen) { return kLdsTileW + 2 * (klen / 2); }
-__host__ __device__ inline int32_t lds_box_stride(int32_t klen) {
-  const int32_t w = lds_box_w(klen);
+__host__ __device__ inline int32_t lds_box_h(int32_t th, int32_t klen) { return th + 2 * (klen / 2); }
+__host__ __device__ inline int32_t lds_box_w(int32_t tw, int32_t klen) { return tw + 2 * (klen / 2); }
+__host__ __device__ inline int32_t lds_box_stride(int32_t tw, int32_t klen) {
+  const int32_t w = lds_box_w(tw, klen);
   return w + ((8 - w) % 32 + 32) % 32;  // == 8 (mod 32)
 }
 
-template <typename T>
-inline size_t lds_replay_bytes(int32_t klen) {
-  return static_cast<size_t>(lds_box_h(klen)) * lds_box_stride(klen) * sizeof(T) +
-         static_cast<size_t>(lds_box_w(klen)) * sizeof(int32_t);
+inline size_t lds_replay_bytes(int32_t th, int32_t tw, int32_t klen, size_t elem) {
+  return static_cast<size_t>(lds_box_h(th, klen)) * lds_box_stride(tw, klen) * elem +
+         static_cast<size_t>(lds_box_w(tw, klen)) * sizeof(int32_t);
 }
 
 __device__ __forceinline__ int32_t wrap_index(int32_t c, int32_t n) {
@@ -327,13 +330,16 @@ __device__ __forceinline__ int32_t wrap_index(int32_t c, int32_t n) {
   return c < 0 ? c + n : c;
 }
 
-// Byte-offset change in the LDS box for one move code.
-__device__ __forceinline__ int32_t lds_move(uint32_t c, int32_t row_bytes, int32_t elem) {
-  const bool xp = c == kMoveXPlus || c == kMoveXWrapHi;
-  const bool xm = c == kMoveXMinus || c == kMoveXWrapLo;
-  const bool yp = c == kMoveYPlus || c == kMoveYWrapHi;
-  const bool ym = c == kMoveYMinus || c == kMoveYWrapLo;
-  return xp ? elem : xm ? -elem : yp ? row_bytes : ym ? -row_bytes : 0;
+// Byte-offset change in the LDS box for one move code: in the box's
+// unwrapped coordinates a periodic edge step is an ordinary step.
+__device__ __forceinline__ int32_t lds_move(uint32_t c, const FieldGeom& g, int32_t row_bytes,
+                                            int32_t elem) {
+  const bool edge = (c & kMoveEdge) != 0 && c != kMoveStop;
+  const uint32_t d = c & 7u;
+  const bool per = d == kMoveXPlus ? g.x_right : d == kMoveXMinus ? g.x_left
+                 : d == kMoveYPlus ? g.y_right : g.y_left;
+  if (c == kMoveStop || d == 0 || d > 4 || (edge && !per)) return 0;
+  return d == kMoveXPlus ? elem : d == kMoveXMinus ? -elem : d == kMoveYPlus ? row_bytes : -row_bytes;
 }
 
 template <typename T>
@@ -341,14 +347,11 @@ __device__ __forceinline__ T lds_at(const unsigned char* smem, uint32_t byte_pos
   return *reinterpret_cast<const T*>(smem + byte_pos);
 }
 
-template <typename T, bool FMA, bool FORWARD>
+template <typename T, bool FMA, bool FORWARD, int E>
 __device__ __forceinline__ void replay_directions_lds(
-    T (&acc)[kLdsPixelsPerLane], const uint32_t (&pos0)[kLdsPixelsPerLane],
-    const bool (&inside)[kLdsPixelsPerLane], const unsigned char* box,
+    T (&acc)[E], const uint32_t (&pos0)[E], const bool (&inside)[E], const unsigned char* box,
     const T* __restrict__ kernel, int32_t kmid, int32_t nsteps, const uint32_t* pair_delta,
-    const uint4* __restrict__ codes, int64_t np, const int64_t (&p)[kLdsPixelsPerLane],
-    int32_t c0) {
-  constexpr int E = kLdsPixelsPerLane;
+    const uint4* __restrict__ codes, int64_t np, const int64_t (&p)[E], int32_t c0) {
   const int32_t nchunks = chunks_of(nsteps);
   bool alive[E];
   uint32_t pos[E];
@@ -408,42 +411,57 @@ __device__ __forceinline__ void replay_directions_lds(
   }
 }
 
-template <typename T, bool FMA>
+template <typename T, bool FMA, int TH, int TW>
 __global__ __launch_bounds__(kLdsThreads) void lic_replay_lds(const T* __restrict__ x,
                                                               const T* __restrict__ kernel,
                                                               int32_t klen, FieldGeom g,
                                                               TileMap m, T* __restrict__ out,
                                                               const uint4* __restrict__ codes) {
+  constexpr int E = TH * TW / kLdsThreads;  // pixels per lane: 1, 2 or 4
   extern __shared__ __align__(16) unsigned char smem[];
-  constexpr int E = kLdsPixelsPerLane;
   const int32_t kmid = klen / 2;
-  const int32_t bh = lds_box_h(klen), bw = lds_box_w(klen), stride = lds_box_stride(klen);
+  const int32_t bh = lds_box_h(TH, klen), bw = lds_box_w(TW, klen);
+  const int32_t stride = lds_box_stride(TW, klen);
   int32_t* col_map = reinterpret_cast<int32_t*>(smem + sizeof(T) * bh * stride);
   __shared__ uint32_t pair_delta[256];
 
   int32_t ty, tx;
   tile_of_block(blockIdx.x, m, ty, tx);
-  const int32_t i_box0 = ty * kLdsTileH - kmid, j_box0 = tx * kLdsTileW - kmid;
+  const int32_t i_box0 = ty * TH - kmid, j_box0 = tx * TW - kmid;
   const int32_t tid = threadIdx.x;
   const int32_t row_bytes = stride * static_cast<int32_t>(sizeof(T));
   if (tid < 256) {
-    const int32_t lo = lds_move(tid & 15, row_bytes, sizeof(T));
-    const int32_t hi = lds_move(tid >> 4, row_bytes, sizeof(T));
+    const int32_t lo = lds_move(tid & 15, g, row_bytes, sizeof(T));
+    const int32_t hi = lds_move(tid >> 4, g, row_bytes, sizeof(T));
     pair_delta[tid] = (static_cast<uint32_t>(hi) << 16) | (static_cast<uint32_t>(lo) & 0xFFFFu);
   }
   for (int32_t b = tid; b < bw; b += kLdsThreads) col_map[b] = wrap_index(j_box0 + b, g.nx);
   __syncthreads();
-  // stage the box: wave w copies rows w, w + 16, ... (coalesced along x)
+  // stage the box: wave w copies rows w, w + 16, ... (coalesced along x), four
+  // loads in flight per lane
   T* box_t = reinterpret_cast<T*>(smem);
   const int32_t wave = tid >> 6, lane = tid & 63;
   for (int32_t a = wave; a < bh; a += kLdsThreads / 64) {
     const int64_t src_row = static_cast<int64_t>(wrap_index(i_box0 + a, g.ny)) * g.nx;
-    for (int32_t b = lane; b < bw; b += 64) box_t[a * stride + b] = x[src_row + col_map[b]];
+    for (int32_t b0 = lane; b0 < bw; b0 += 4 * 64) {
+      T v[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int32_t b = b0 + 64 * t;
+        v[t] = b < bw ? x[src_row + col_map[b]] : T(0);
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int32_t b = b0 + 64 * t;
+        if (b < bw) box_t[a * stride + b] = v[t];
+      }
+    }
   }
   __syncthreads();
 
-  // wave w covers the 8 x 8 squares at (8 * (w / 4), 8 * (w % 4)) and 32 columns to the right
-  const int32_t py = ((wave >> 2) << 3) | (lane >> 3);
+  // wave w covers the 8 x 8 square at (8 * (w / 4), 8 * (w % 4)) of each
+  // 32 x 32 quarter e of the tile
+  const int32_t py0 = ((wave >> 2) << 3) | (lane >> 3);
   const int32_t px0 = ((wave & 3) << 3) | (lane & 7);
   const int32_t n_fwd = klen - 1 - kmid, n_bwd = kmid;
   const int64_t np = static_cast<int64_t>(g.ny) * g.nx;
@@ -453,17 +471,17 @@ __global__ __launch_bounds__(kLdsThreads) void lic_replay_lds(const T* __restric
   int64_t p[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const int32_t px = px0 + 32 * e;
-    const int32_t i0 = ty * kLdsTileH + py, j0 = tx * kLdsTileW + px;
+    const int32_t py = py0 + 32 * (e / (TW / 32)), px = px0 + 32 * (e % (TW / 32));
+    const int32_t i0 = ty * TH + py, j0 = tx * TW + px;
     inside[e] = i0 < g.ny && j0 < g.nx;
     p[e] = inside[e] ? static_cast<int64_t>(i0) * g.nx + j0 : 0;
     pos0[e] = static_cast<uint32_t>(((kmid + py) * stride + (kmid + px)) * sizeof(T));
     acc[e] = macc<FMA>(kernel[kmid], lds_at<T>(smem, pos0[e]), T(0));
   }
-  replay_directions_lds<T, FMA, true>(acc, pos0, inside, smem, kernel, kmid, n_fwd, pair_delta,
-                                      codes, np, p, 0);
-  replay_directions_lds<T, FMA, false>(acc, pos0, inside, smem, kernel, kmid, n_bwd, pair_delta,
-                                       codes, np, p, chunks_of(n_fwd));
+  replay_directions_lds<T, FMA, true, E>(acc, pos0, inside, smem, kernel, kmid, n_fwd,
+                                         pair_delta, codes, np, p, 0);
+  replay_directions_lds<T, FMA, false, E>(acc, pos0, inside, smem, kernel, kmid, n_bwd,
+                                          pair_delta, codes, np, p, chunks_of(n_fwd));
 #pragma unroll
   for (int e = 0; e < E; ++e)
     if (inside[e]) out[p[e]] = acc[e];
@@ -610,35 +628,63 @@ int32_t lds_replay_limit_bytes() {
   return limit;
 }
 
+// LDS replay tile for this kernel length: the largest of 64x64, 32x64 and
+// 32x32 whose staged box fits the per-block LDS; {0, 0} when none does.
+template <typename T>
+LdsTile lds_replay_tile(int32_t klen) {
+  const size_t limit = static_cast<size_t>(lds_replay_limit_bytes());
+  const int32_t shapes[3][2] = {{64, 64}, {32, 64}, {32, 32}};
+  for (const auto& sh : shapes) {
+    // 1 KB of static LDS (pair table) + margin
+    if (lds_replay_bytes(sh[0], sh[1], klen, sizeof(T)) + 2048 <= limit) return {sh[0], sh[1]};
+  }
+  return {0, 0};
+}
+
 template <typename T>
 bool lds_replay_fits(int32_t klen) {
-  return lds_replay_bytes<T>(klen) + 64 <= static_cast<size_t>(lds_replay_limit_bytes());
+  return lds_replay_tile<T>(klen).h > 0;
+}
+
+template <typename T, bool FMA, int TH, int TW>
+static void replay_lds_one(const T* x, const T* kernel, int32_t klen, const FieldGeom& g, T* out,
+                           const uint4* codes, hipStream_t stream) {
+  TileMap m;
+  m.wave_w_log2 = 3;
+  m.tiles_x = (g.nx + TW - 1) / TW;
+  m.tiles_y = (g.ny + TH - 1) / TH;
+  m.n_tiles = m.tiles_x * m.tiles_y;
+  m.per_xcd = m.n_tiles / kXcds;
+  m.rem_xcd = m.n_tiles % kXcds;
+  m.band = 4;
+  const size_t bytes = lds_replay_bytes(TH, TW, klen, sizeof(T));
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&lic_replay_lds<T, FMA, TH, TW>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  (void)hipGetLastError();  // the attribute is advisory on this target
+  hipLaunchKernelGGL((lic_replay_lds<T, FMA, TH, TW>), dim3(m.n_tiles), dim3(kLdsThreads), bytes,
+                     stream, x, kernel, klen, g, m, out, codes);
+}
+
+template <typename T, bool FMA>
+static void replay_lds_shape(const T* x, const T* kernel, int32_t klen, const FieldGeom& g,
+                             T* out, const uint4* codes, hipStream_t stream) {
+  const LdsTile t = lds_replay_tile<T>(klen);
+  if (t.h == 64) {
+    replay_lds_one<T, FMA, 64, 64>(x, kernel, klen, g, out, codes, stream);
+  } else if (t.w == 64) {
+    replay_lds_one<T, FMA, 32, 64>(x, kernel, klen, g, out, codes, stream);
+  } else {
+    replay_lds_one<T, FMA, 32, 32>(x, kernel, klen, g, out, codes, stream);
+  }
 }
 
 template <typename T>
 hipError_t launch_replay_lds(const T* x, const T* kernel, int32_t klen, const FieldGeom& g,
                              int profile, T* out, const uint4* codes, hipStream_t stream) {
-  TileMap m;
-  m.wave_w_log2 = 4;  // unused by this kernel except through tile_of_block
-  m.tiles_x = (g.nx + kLdsTileW - 1) / kLdsTileW;
-  m.tiles_y = (g.ny + kLdsTileH - 1) / kLdsTileH;
-  m.n_tiles = m.tiles_x * m.tiles_y;
-  m.per_xcd = m.n_tiles / kXcds;
-  m.rem_xcd = m.n_tiles % kXcds;
-  m.band = 4;
-  const size_t bytes = lds_replay_bytes<T>(klen);
   if (profile & 1) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&lic_replay_lds<T, true>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    (void)hipGetLastError();  // the attribute is advisory on this target
-    hipLaunchKernelGGL((lic_replay_lds<T, true>), dim3(m.n_tiles), dim3(kLdsThreads), bytes,
-                       stream, x, kernel, klen, g, m, out, codes);
+    replay_lds_shape<T, true>(x, kernel, klen, g, out, codes, stream);
   } else {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&lic_replay_lds<T, false>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    (void)hipGetLastError();  // the attribute is advisory on this target
-    hipLaunchKernelGGL((lic_replay_lds<T, false>), dim3(m.n_tiles), dim3(kLdsThreads), bytes,
-                       stream, x, kernel, klen, g, m, out, codes);
+    replay_lds_shape<T, false>(x, kernel, klen, g, out, codes, stream);
   }
   return hipGetLastError();
 }
@@ -656,6 +702,7 @@ hipError_t launch_replay_lds(const T* x, const T* kernel, int32_t klen, const Fi
   template hipError_t launch_replay<T>(const T*, const T*, int32_t, const FieldGeom&,          \
                                        const TileMap&, int, T*, const uint4*, hipStream_t);    \
   template bool lds_replay_fits<T>(int32_t);                                                   \
+  template LdsTile lds_replay_tile<T>(int32_t);                                                \
   template hipError_t launch_replay_lds<T>(const T*, const T*, int32_t, const FieldGeom&, int, \
                                            T*, const uint4*, hipStream_t);
 

@@ -48,8 +48,13 @@ hipError_t launch_replay(const T* x, const T* kernel, int32_t klen, const FieldG
                          const TileMap& m, int profile, T* out, const uint4* codes,
                          hipStream_t stream);
 
-// LDS-staged replay (32 x 32 tiles): usable when the tile plus its kmid
-// halo fits the device's per-block LDS.
+// LDS-staged replay: tiles of 64x64, 32x64 or 32x32 pixels, the largest
+// whose box (tile plus kmid halo) fits the device's per-block LDS.
+struct LdsTile {
+  int32_t h, w;
+};
+template <typename T>
+LdsTile lds_replay_tile(int32_t klen);
 template <typename T>
 bool lds_replay_fits(int32_t klen);
 

@@ -285,3 +285,23 @@ def test_replay_paths_match(lds, dtype, monkeypatch):
             exp = cpu.convolve(tex, u, v, kernel=k, boundaries=b, iterations=3)
             got = rlic_amd.convolve(tex, u, v, kernel=k, boundaries=b, iterations=3)
             _assert_same(got, exp, f"lds={lds} {shape} L={klen} {b}")
+
+
+@pytest.mark.parametrize("dtype,klen", [
+    (np.float32, 65),   # 64x64 LDS tiles
+    (np.float32, 137),  # 32x64 LDS tiles
+    (np.float32, 161),  # 32x32 LDS tiles
+    (np.float32, 181),  # box too large for LDS: global gathers
+    (np.float64, 73),   # 64x64
+    (np.float64, 81),   # 32x64
+])
+def test_replay_tile_shapes(dtype, klen):
+    rng = np.random.default_rng(klen)
+    tex, u, v = _fields(rng, (70, 130), dtype, "holes")
+    k = rng.random(klen).astype(dtype)
+    for b in ("periodic", {"x": ("closed", "closed"), "y": "periodic"}):
+        exp = cpu.convolve(tex, u, v, kernel=k, uv_mode="polarization", boundaries=b,
+                           iterations=2)
+        got = rlic_amd.convolve(tex, u, v, kernel=k, uv_mode="polarization", boundaries=b,
+                                iterations=2)
+        _assert_same(got, exp, f"{np.dtype(dtype).name} L={klen} {b}")
