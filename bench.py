@@ -86,6 +86,9 @@ def main() -> int:
     ap.add_argument("--workload", default="plain4096")
     ap.add_argument("--cpu-rows", type=int, default=2048,
                     help="rows of pass 1 timed on the CPU oracle (0 = skip)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="N",
+                    help="first run the library's copy kernel N times on a 1 GiB array "
+                         "(known HBM bytes, for calibrating FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -102,6 +105,17 @@ def main() -> int:
 
     assert torch.cuda.is_available(), "no ROCm device visible"
     dev = torch.device("cuda:0")
+    if args.calibrate_copy > 0:
+        # lic_copy2d reads and writes 2^28 float32 = 1 GiB each way per call
+        src = torch.rand((16384, 16384), device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(args.calibrate_copy):
+            _core.convolve_f32(src, (src, src, "velocity"), torch.ones(1, device=dev),
+                               (("closed",) * 2,) * 2, 0)
+        torch.cuda.synchronize()
+        log(f"bench: {args.calibrate_copy} calibration copies of 1 GiB done")
+        del src, dst
+        torch.cuda.empty_cache()
     log(f"bench: building workload {args.workload}")
     wl = WORKLOADS[args.workload]()
     fn = _core.convolve_f32 if wl.texture.dtype == np.float32 else _core.convolve_f64

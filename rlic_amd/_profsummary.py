@@ -1,0 +1,121 @@
+"""Summaries of rocprofv3 CSV output for the LIC kernels.
+
+    python -m rlic_amd._profsummary stats  <kernel_stats.csv>
+    python -m rlic_amd._profsummary traffic <FETCH_SIZE counter_collection.csv>
+                                            <WRITE_SIZE counter_collection.csv>
+                                            [--workload NAME] [--out-dir DIR]
+
+`stats` prints the per-kernel launch count and average duration.
+`traffic` turns the per-dispatch FETCH_SIZE / WRITE_SIZE counters (KiB)
+into HBM bytes per launch.  Each counter is first calibrated on the
+library's copy kernel (lic_copy2d, run by ``bench.py --calibrate-copy`` on
+2^28 float32 = 1 GiB read and 1 GiB written per dispatch): factor = known
+bytes / counter bytes.  With --out-dir, writes traffic_<kernel>_<workload>.json
+files that bench.py reports as roofline.traffic.
+"""
+
+from __future__ import annotations
+
+import argparse
+import csv
+import json
+import re
+import sys
+from collections import defaultdict
+from pathlib import Path
+
+COPY_KERNEL = "lic_copy2d"
+COPY_BYTES = 2**28 * 4  # one lic_copy2d dispatch of the calibration array, each way
+
+
+def short_name(full: str) -> str:
+    """'void rlic::lic_trace<float, ...>(...)' -> 'lic_trace'."""
+    m = re.search(r"rlic::(\w+)", full)
+    return m.group(1) if m else full.split("(")[0][:60]
+
+
+def read_stats(path: Path) -> dict[str, dict]:
+    out: dict[str, dict] = {}
+    with open(path, newline="") as f:
+        for row in csv.DictReader(f):
+            name = short_name(row["Name"])
+            agg = out.setdefault(name, {"calls": 0, "total_ns": 0.0})
+            agg["calls"] += int(row["Calls"])
+            agg["total_ns"] += float(row["TotalDurationNs"])
+    for agg in out.values():
+        agg["avg_ms"] = agg["total_ns"] / agg["calls"] / 1e6
+    return out
+
+
+def read_counter(path: Path, counter: str) -> dict[str, list[float]]:
+    """Per kernel: the counter's value (summed over instances) per dispatch."""
+    per_dispatch: dict[tuple[str, str], float] = defaultdict(float)
+    with open(path, newline="") as f:
+        for row in csv.DictReader(f):
+            if row["Counter_Name"] != counter:
+                continue
+            key = (short_name(row["Kernel_Name"]), row["Dispatch_Id"])
+            per_dispatch[key] += float(row["Counter_Value"])
+    out: dict[str, list[float]] = defaultdict(list)
+    for (name, _), value in per_dispatch.items():
+        out[name].append(value)
+    return out
+
+
+def traffic(fetch_csv: Path, write_csv: Path) -> dict[str, dict]:
+    fetch = read_counter(fetch_csv, "FETCH_SIZE")
+    write = read_counter(write_csv, "WRITE_SIZE")
+    if COPY_KERNEL not in fetch or COPY_KERNEL not in write:
+        raise SystemExit(f"no {COPY_KERNEL} dispatches: run bench.py with --calibrate-copy")
+    avg = lambda xs: sum(xs) / len(xs)  # noqa: E731
+    f_cal = COPY_BYTES / (avg(fetch[COPY_KERNEL]) * 1024)
+    w_cal = COPY_BYTES / (avg(write[COPY_KERNEL]) * 1024)
+    result = {}
+    for name in sorted(set(fetch) & set(write)):
+        if not name.startswith("lic_"):
+            continue
+        fb = avg(fetch[name]) * 1024 * f_cal
+        wb = avg(write[name]) * 1024 * w_cal
+        result[name] = {
+            "fetch_bytes_per_launch": fb,
+            "write_bytes_per_launch": wb,
+            "hbm_bytes_per_launch": fb + wb,
+            "fetch_kib_raw": avg(fetch[name]),
+            "write_kib_raw": avg(write[name]),
+            "dispatches": len(fetch[name]),
+            "fetch_calibration": f_cal,
+            "write_calibration": w_cal,
+        }
+    return result
+
+
+def main(argv: list[str] | None = None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    s = sub.add_parser("stats")
+    s.add_argument("csv", type=Path)
+    t = sub.add_parser("traffic")
+    t.add_argument("fetch_csv", type=Path)
+    t.add_argument("write_csv", type=Path)
+    t.add_argument("--workload", default="plain4096")
+    t.add_argument("--out-dir", type=Path)
+    args = ap.parse_args(argv)
+    if args.cmd == "stats":
+        for name, agg in sorted(read_stats(args.csv).items(), key=lambda kv: -kv[1]["total_ns"]):
+            print(f"{name:24s} calls {agg['calls']:6d}  avg {agg['avg_ms']:10.4f} ms  "
+                  f"total {agg['total_ns'] / 1e6:10.3f} ms")
+        return 0
+    res = traffic(args.fetch_csv, args.write_csv)
+    for name, r in res.items():
+        print(f"{name:24s} HBM {r['hbm_bytes_per_launch'] / 1e9:9.3f} GB/launch "
+              f"(fetch {r['fetch_bytes_per_launch'] / 1e9:.3f}, write "
+              f"{r['write_bytes_per_launch'] / 1e9:.3f}; calibration x{r['fetch_calibration']:.3f}"
+              f" / x{r['write_calibration']:.3f})")
+        if args.out_dir:
+            path = args.out_dir / f"traffic_{name}_{args.workload}.json"
+            path.write_text(json.dumps(r, indent=1) + "\n")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
