@@ -78,6 +78,73 @@ def cpu_baseline(wl, rows: int) -> dict:
     }
 
 
+def bench_batch(args, dev) -> int:
+    """512 uneven fields: one persistent batched launch vs a per-call loop."""
+    import torch
+
+    from rlic_amd import _core, _native
+    from rlic_amd.batch import convolve_batch
+    from rlic_amd.workloads import batch_uneven, pixel_steps
+
+    log("bench: generating 512 uneven fields on the device")
+    fields = batch_uneven(512, seed=4, device=dev)
+    steps_total = pixel_steps(fields)
+    bpairs = []
+    for f in fields:
+        bs = __import__("rlic_amd")._boundaries.BoundarySet.from_spec(f["boundaries"])
+        bpairs.append((bs.x, bs.y))
+
+    def per_call():
+        for f, bp in zip(fields, bpairs):
+            _core.convolve_f32(f["texture"], (f["u"], f["v"], f["uv_mode"]), f["kernel"], bp,
+                               f["iterations"])
+
+    def batched():
+        convolve_batch(fields)
+
+    timings = {}
+    for name, fn in (("per_call", per_call), ("batched", batched)):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        _native.profiler_enable(name == "batched")
+        _native.profiler_read()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        timings[name] = (time.perf_counter() - t0) / args.steps
+        log(f"bench: {name} {timings[name] * 1e3:.2f} ms per batch")
+    prof = _native.profiler_read()
+    _native.profiler_enable(False)
+    value = steps_total / timings["batched"] / 1e9
+    per_call_value = steps_total / timings["per_call"] / 1e9
+    line = {
+        "metric": "Gpixel-steps/s (batched uneven fields)",
+        "value": round(value, 3),
+        "unit": "Gpixel-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(timings["batched"] * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "batch_uneven_512_f32", "fields": len(fields),
+                   "pixel_steps_per_step": steps_total},
+        "per_call": {"value": round(per_call_value, 3),
+                     "ms_per_step": round(timings["per_call"] * 1e3, 3)},
+        "batched_vs_per_call_speedup": round(timings["per_call"] / timings["batched"], 3),
+        "kernel_ms_per_step": {k: round(v["total_ms"] / args.steps, 3) for k, v in prof.items()},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +183,8 @@ def main() -> int:
         log(f"bench: {args.calibrate_copy} calibration copies of 1 GiB done")
         del src, dst
         torch.cuda.empty_cache()
+    if args.workload == "batch512":
+        return bench_batch(args, dev)
     log(f"bench: building workload {args.workload}")
     wl = WORKLOADS[args.workload]()
     fn = _core.convolve_f32 if wl.texture.dtype == np.float32 else _core.convolve_f64

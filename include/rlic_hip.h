@@ -98,6 +98,41 @@ int rlic_convolve_f64(const double* texture, int64_t tex_sy, int64_t tex_sx,
                       double* out,
                       int location, int profile, void* stream);
 
+/* One field of a batched call: the arguments of one rlic_convolve_* call
+ * with DEVICE pointers (see rlic_convolve_batch_*). */
+typedef struct {
+  const float* texture; int64_t tex_sy, tex_sx;
+  const float* u; int64_t u_sy, u_sx;
+  const float* v; int64_t v_sy, v_sx;
+  int64_t ny, nx;
+  const float* kernel; int64_t kernel_len;
+  int uv_mode, x_left, x_right, y_left, y_right;
+  int64_t iterations;
+  float* out;
+} rlic_field_f32;
+
+typedef struct {
+  const double* texture; int64_t tex_sy, tex_sx;
+  const double* u; int64_t u_sy, u_sx;
+  const double* v; int64_t v_sy, v_sx;
+  int64_t ny, nx;
+  const double* kernel; int64_t kernel_len;
+  int uv_mode, x_left, x_right, y_left, y_right;
+  int64_t iterations;
+  double* out;
+} rlic_field_f64;
+
+/* Batched convolution: n_fields independent rlic_convolve_* calls (device
+ * memory only) run together in one persistent launch fed by a device-side
+ * work queue.  Results are the same, bit for bit, as the single calls.
+ * Returns after the work has completed on `stream`.  No counterpart in the
+ * reference (which has one call per field, src/lib.rs:450-482); each field
+ * follows the semantics of that call. */
+int rlic_convolve_batch_f32(const rlic_field_f32* fields, int64_t n_fields, int profile,
+                            void* stream);
+int rlic_convolve_batch_f64(const rlic_field_f64* fields, int64_t n_fields, int profile,
+                            void* stream);
+
 /* Message for the calling thread's most recent failed call ("" if none).
  * No counterpart in the reference, whose errors abort (Cargo.toml:23). */
 const char* rlic_last_error(void);

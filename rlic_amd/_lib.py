@@ -18,7 +18,7 @@ tensor.
 
 from __future__ import annotations
 
-__all__ = ["convolve"]
+__all__ = ["convolve", "validate"]
 
 import sys
 from typing import TYPE_CHECKING, Any
@@ -139,6 +139,22 @@ def convolve(
     ExceptionGroup
       When more than one problem is found; all are reported.
     """
+    bs, dtype = validate(
+        texture, u, v, kernel=kernel, uv_mode=uv_mode, boundaries=boundaries, iterations=iterations
+    )
+    if iterations == 0:
+        return _copy(texture)
+
+    native = convolve_f32 if dtype == np.dtype("float32") else convolve_f64
+    return native(texture, (u, v, uv_mode), kernel, (bs.x, bs.y), iterations)
+
+
+def validate(texture, u, v, *, kernel, uv_mode, boundaries, iterations):
+    """Run every input check of ``convolve`` (src/rlic/_lib.py:135-205).
+
+    Raises the single problem found, or all of them as an ExceptionGroup;
+    returns the normalised BoundarySet and the common dtype otherwise.
+    """
     problems: list[Exception] = []
     if iterations < 0:
         problems.append(
@@ -200,10 +216,5 @@ def convolve(
         raise problems[0]
     if problems:
         raise _ExceptionGroup("Invalid inputs were received.", problems)
-
     assert bs is not None
-    if iterations == 0:
-        return _copy(texture)
-
-    native = convolve_f32 if dtypes[0] == np.dtype("float32") else convolve_f64
-    return native(texture, (u, v, uv_mode), kernel, (bs.x, bs.y), iterations)
+    return bs, dtypes[0]

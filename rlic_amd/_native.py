@@ -33,7 +33,28 @@ _CONVOLVE_ARGTYPES = [
     _int, _int, _vp,  # location, profile, stream
 ]
 
+def _field_struct(name: str, ctype) -> type:
+    """ctypes mirror of rlic_field_f32 / rlic_field_f64 (include/rlic_hip.h)."""
+    ptr = ctypes.POINTER(ctype)
+    return type(name, (ctypes.Structure,), {"_fields_": [
+        ("texture", ptr), ("tex_sy", _i64), ("tex_sx", _i64),
+        ("u", ptr), ("u_sy", _i64), ("u_sx", _i64),
+        ("v", ptr), ("v_sy", _i64), ("v_sx", _i64),
+        ("ny", _i64), ("nx", _i64),
+        ("kernel", ptr), ("kernel_len", _i64),
+        ("uv_mode", _int), ("x_left", _int), ("x_right", _int), ("y_left", _int),
+        ("y_right", _int),
+        ("iterations", _i64),
+        ("out", ptr),
+    ]})
+
+
+FieldF32 = _field_struct("FieldF32", ctypes.c_float)
+FieldF64 = _field_struct("FieldF64", ctypes.c_double)
+
 SIGNATURES: dict[str, tuple[list, object]] = {
+    "rlic_convolve_batch_f32": ([ctypes.POINTER(FieldF32), _i64, _int, _vp], _int),
+    "rlic_convolve_batch_f64": ([ctypes.POINTER(FieldF64), _i64, _int, _vp], _int),
     "rlic_convolve_f32": (_CONVOLVE_ARGTYPES, _int),
     "rlic_convolve_f64": (_CONVOLVE_ARGTYPES, _int),
     "rlic_last_error": ([], ctypes.c_char_p),

@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "../../include/rlic_hip.h"
+#include "lic_batch.h"
 #include "lic_launch.h"
 
 namespace {
@@ -242,6 +243,76 @@ int run_on_device(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
   return RLIC_OK;
 }
 
+// Argument checks shared by the single and batched entry points.
+template <typename T>
+int validate(const T* tex, const T* u, const T* v, int64_t ny, int64_t nx, const T* kernel,
+             int64_t klen, int uv_mode, int xl, int xr, int yl, int yr, int64_t iterations,
+             const T* out) {
+  if (ny < 0 || nx < 0) return set_error(RLIC_ERR_INVALID_ARGUMENT, "negative image size");
+  if (klen < 0) return set_error(RLIC_ERR_INVALID_ARGUMENT, "negative kernel length");
+  if (iterations < 0)
+    return set_error(RLIC_ERR_INVALID_ARGUMENT, "Invalid number of iterations: %lld",
+                     static_cast<long long>(iterations));
+  if (uv_mode != RLIC_UV_VELOCITY && uv_mode != RLIC_UV_POLARIZATION)
+    return set_error(RLIC_ERR_INVALID_ARGUMENT, "unknown uv_mode %d", uv_mode);
+  for (int b : {xl, xr, yl, yr}) {
+    if (b != RLIC_BOUND_CLOSED && b != RLIC_BOUND_PERIODIC)
+      return set_error(RLIC_ERR_INVALID_ARGUMENT, "unknown boundary %d", b);
+  }
+  const int64_t n = ny * nx;
+  if (n == 0) return RLIC_OK;
+  if (ny > INT32_MAX || nx > INT32_MAX || n > INT32_MAX || klen > INT32_MAX)
+    return set_error(RLIC_ERR_INVALID_ARGUMENT,
+                     "image of %lld x %lld pixels exceeds the supported 2^31-1 pixels",
+                     static_cast<long long>(ny), static_cast<long long>(nx));
+  if (!tex || !u || !v || !out) return set_error(RLIC_ERR_INVALID_ARGUMENT, "null array pointer");
+  if (iterations > 0 && klen == 0)
+    return set_error(RLIC_ERR_EMPTY_KERNEL,
+                     "kernel is empty: kernel[len(kernel) // 2] does not exist");
+  if (klen > 0 && !kernel) return set_error(RLIC_ERR_INVALID_ARGUMENT, "null kernel pointer");
+  return RLIC_OK;
+}
+
+template <typename T, typename FieldSpec>
+int batch_entry(const FieldSpec* fields, int64_t nfields, int profile, void* stream_ptr) {
+  t_last_error.clear();
+  if (nfields < 0 || (nfields > 0 && !fields))
+    return set_error(RLIC_ERR_INVALID_ARGUMENT, "invalid field list");
+  if (profile < 0 || profile > 3)
+    return set_error(RLIC_ERR_INVALID_ARGUMENT, "unknown arithmetic profile %d", profile);
+  if (nfields > INT32_MAX) return set_error(RLIC_ERR_INVALID_ARGUMENT, "too many fields");
+  for (int64_t k = 0; k < nfields; ++k) {
+    const FieldSpec& f = fields[k];
+    const int rc = validate<T>(f.texture, f.u, f.v, f.ny, f.nx, f.kernel, f.kernel_len,
+                               f.uv_mode, f.x_left, f.x_right, f.y_left, f.y_right,
+                               f.iterations, f.out);
+    if (rc != RLIC_OK) {
+      t_last_error = "field " + std::to_string(k) + ": " + t_last_error;
+      return rc;
+    }
+    if (f.iterations > INT32_MAX)
+      return set_error(RLIC_ERR_INVALID_ARGUMENT, "field %lld: too many iterations",
+                       static_cast<long long>(k));
+  }
+  if (nfields == 0) return RLIC_OK;
+  int dev = 0;
+  RLIC_HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+  configure_pool(dev);
+  hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+  rlic::BatchScratch scratch;
+  std::string err;
+  int rc;
+  {
+    LaunchTimer timer("lic_batch", stream);
+    rc = rlic::batch_run<T, FieldSpec>(fields, nfields, profile, stream, scratch, err);
+  }
+  if (rc != RLIC_OK) {
+    (void)hipStreamSynchronize(stream);
+    return set_error(rc, "%s", err.c_str());
+  }
+  return RLIC_OK;
+}
+
 template <typename T>
 int convolve_entry(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t usy, int64_t usx,
                    const T* v, int64_t vsy, int64_t vsx, int64_t ny, int64_t nx, const T* kernel,
@@ -348,6 +419,16 @@ int rlic_convolve_f64(const double* texture, int64_t tex_sy, int64_t tex_sx, con
   return convolve_entry<double>(texture, tex_sy, tex_sx, u, u_sy, u_sx, v, v_sy, v_sx, ny, nx,
                                 kernel, kernel_len, uv_mode, x_left, x_right, y_left, y_right,
                                 iterations, out, location, profile, stream);
+}
+
+int rlic_convolve_batch_f32(const rlic_field_f32* fields, int64_t n_fields, int profile,
+                            void* stream) {
+  return batch_entry<float, rlic_field_f32>(fields, n_fields, profile, stream);
+}
+
+int rlic_convolve_batch_f64(const rlic_field_f64* fields, int64_t n_fields, int profile,
+                            void* stream) {
+  return batch_entry<double, rlic_field_f64>(fields, n_fields, profile, stream);
 }
 
 const char* rlic_last_error(void) { return t_last_error.c_str(); }

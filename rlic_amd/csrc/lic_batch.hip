@@ -1,0 +1,466 @@
+// Batched convolution of many independent fields in one persistent launch.
+//
+// Each field is one `rlic.convolve` call (src/lib.rs:408-443) with its own
+// size, kernel, boundaries, uv_mode and iteration count.  Work items are
+// 32 x 32 pixel regions of one pass of one field, queued pass-major: every
+// field's pass 1, then every field's pass 2, and so on.  The waves of a
+// persistent grid pop items from one atomic counter.  A pass-p item of a
+// field waits until the field's pass p-1 is complete; the items it waits for
+// were queued earlier, so they were popped by waves that are already running
+// and whose own waits point further back, hence every wait ends (a wait longer than a
+// time limit still ends, and makes the call fail).  Pass 1 traces (and
+// records moves when more passes follow), later passes replay the moves
+// with gathers through L2.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/rlic_hip.h"
+#include "lic_batch.h"
+#include "lic_device.h"
+#include "lic_launch.h"
+#include "lic_steps.h"
+
+namespace rlic {
+
+constexpr int32_t kRegion = 32;  // pixels per side of a work item (one wave)
+
+template <typename T>
+struct BatchField {
+  const T* tex;
+  int64_t tsy, tsx;
+  const T* u;
+  int64_t usy, usx;
+  const T* v;
+  int64_t vsy, vsx;
+  const T* kernel;  // first weight of the zero-padded copy
+  T* out;           // last pass writes here
+  T* tmp;           // ping-pong partner (passes > 1)
+  Cell<T>* cells;
+  uint4* codes;     // move record (passes > 1)
+  FieldGeom g;
+  int32_t klen, uv_mode, iterations;
+  int32_t regions_x, regions;
+  int64_t pack_block0;  // first block of this field in the batched pack grid
+};
+
+struct Segment {
+  int64_t item0;   // first queue item of this (pass, field)
+  int32_t field, pass;
+  int32_t prev;    // segment of the same field's previous pass, -1 for pass 1
+  int32_t pad;
+};
+
+template <typename T>
+__device__ __forceinline__ T* pass_buffer(const BatchField<T>& f, int32_t pass) {
+  // the last pass lands in `out`; passes alternate backwards from there
+  return ((f.iterations - 1 - pass) % 2 == 0) ? f.out : f.tmp;
+}
+
+// Kernel weights of every field into its zero-padded slot (one block per field).
+template <typename T>
+__global__ __launch_bounds__(256) void lic_pad_batch(const BatchField<T>* fields,
+                                                     const T* const* kernels) {
+  const BatchField<T>& f = fields[blockIdx.x];
+  const T* src = kernels[blockIdx.x];
+  T* dst = const_cast<T*>(f.kernel) - kKernelPad;
+  const int32_t total = f.klen + 2 * kKernelPad;
+  for (int32_t t = threadIdx.x; t < total; t += 256) {
+    const int32_t k = t - kKernelPad;
+    dst[t] = (k >= 0 && k < f.klen) ? src[k] : T(0);
+  }
+}
+
+// Pack every field's (texture, u, v) into cells; a field with zero passes
+// gets its texture copied to `out` instead.  Block b covers 64 x 4 pixels of
+// the field whose pack_block0 range contains b.
+template <typename T>
+__global__ __launch_bounds__(256) void lic_pack_batch(const BatchField<T>* fields, int32_t nfields) {
+  const int64_t b = blockIdx.x;
+  int32_t lo = 0, hi = nfields - 1;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi + 1) >> 1;
+    if (fields[mid].pack_block0 <= b) lo = mid; else hi = mid - 1;
+  }
+  const BatchField<T>& f = fields[lo];
+  const int64_t local = b - f.pack_block0;
+  const int32_t bx = (f.g.nx + 63) / 64;
+  const int32_t j = static_cast<int32_t>(local % bx) * 64 + (threadIdx.x & 63);
+  const int32_t i = static_cast<int32_t>(local / bx) * 4 + (threadIdx.x >> 6);
+  if (i >= f.g.ny || j >= f.g.nx) return;
+  const T x = f.tex[i * f.tsy + j * f.tsx];
+  const int64_t idx = static_cast<int64_t>(i) * f.g.nx + j;
+  if (f.iterations == 0) {
+    f.out[idx] = x;
+    return;
+  }
+  Cell<T> c;
+  c.u = f.u[i * f.usy + j * f.usx];
+  c.v = f.v[i * f.vsy + j * f.vsx];
+  c.x = x;
+  c.pad = T(0);
+  f.cells[idx] = c;
+}
+
+__global__ void lic_zero_u32(uint32_t* p, int64_t n) {
+  for (int64_t t = blockIdx.x * 256 + threadIdx.x; t < n; t += gridDim.x * 256) p[t] = 0u;
+}
+
+template <typename T, bool FMA, bool BRANCHLESS, int POL, int REC, bool BUF>
+__device__ __forceinline__ void batch_trace_pixel(const BatchField<T>& f, int32_t i0, int32_t j0) {
+  const CellGather<T, BUF> cell(f.cells, static_cast<int64_t>(f.g.ny) * f.g.nx);
+  const StepGeom sg = make_step_geom(f.g, cell.unit);
+  const int32_t kmid = f.klen / 2;
+  const int32_t n_fwd = f.klen - 1 - kmid, n_bwd = kmid;
+  const int64_t np = static_cast<int64_t>(f.g.ny) * f.g.nx;
+  const int64_t p = static_cast<int64_t>(i0) * f.g.nx + j0;
+  const uint32_t row0 = static_cast<uint32_t>(i0) * sg.nx;
+  const uint32_t jj0 = static_cast<uint32_t>(j0) * cell.unit;
+  const CellVal<T> c0 = cell(row0 + jj0);
+  T acc = macc<FMA>(f.kernel[kmid], c0.x, T(0));  // src/lib.rs:375-383
+  acc = trace_direction<T, FMA, BRANCHLESS, POL, true, REC, BUF>(
+      acc, c0.u, c0.v, row0, jj0, cell, f.kernel, kmid, n_fwd, sg, f.codes, np, p, 0);
+  acc = trace_direction<T, FMA, BRANCHLESS, POL, false, REC, BUF>(
+      acc, c0.u, c0.v, row0, jj0, cell, f.kernel, kmid, n_bwd, sg, f.codes, np, p,
+      chunks_of(n_fwd));
+  pass_buffer(f, 0)[p] = acc;
+}
+
+template <typename T, bool FMA, bool BUF>
+__device__ __forceinline__ void batch_replay_pixel(const BatchField<T>& f, int32_t pass, const uint32_t* delta,
+                                   int32_t i0, int32_t j0) {
+  const ValueGather<T, BUF> x(pass_buffer(f, pass - 1), static_cast<int64_t>(f.g.ny) * f.g.nx);
+  const int32_t kmid = f.klen / 2;
+  const int32_t n_fwd = f.klen - 1 - kmid, n_bwd = kmid;
+  const int64_t np = static_cast<int64_t>(f.g.ny) * f.g.nx;
+  const int64_t idx0 = static_cast<int64_t>(i0) * f.g.nx + j0;
+  const uint32_t pos0 = static_cast<uint32_t>(idx0) * x.unit;
+  T acc = macc<FMA>(f.kernel[kmid], x(pos0), T(0));
+  acc = replay_direction<T, FMA, true, BUF>(acc, pos0, x, f.kernel, kmid, n_fwd, delta, f.codes,
+                                            np, idx0, 0);
+  acc = replay_direction<T, FMA, false, BUF>(acc, pos0, x, f.kernel, kmid, n_bwd, delta, f.codes,
+                                             np, idx0, chunks_of(n_fwd));
+  pass_buffer(f, pass)[idx0] = acc;
+}
+
+// The 32 x 32 region of a work item as 16 squares of 8 x 8 lanes; (i_base,
+// j_base) is the lane's pixel in the first square.
+template <typename T, bool FMA, bool BRANCHLESS, int POL, int REC, bool BUF>
+__device__ __forceinline__ void batch_trace_region(const BatchField<T>& f, int32_t i_base,
+                                                   int32_t j_base) {
+#pragma unroll 1
+  for (int32_t st = 0; st < 16; ++st) {
+    const int32_t i0 = i_base + (st >> 2) * 8, j0 = j_base + (st & 3) * 8;
+    if (i0 < f.g.ny && j0 < f.g.nx) batch_trace_pixel<T, FMA, BRANCHLESS, POL, REC, BUF>(f, i0, j0);
+  }
+}
+
+template <typename T, bool FMA, bool BUF>
+__device__ __forceinline__ void batch_replay_region(const BatchField<T>& f, int32_t pass,
+                                                    const uint32_t* delta, int32_t i_base,
+                                                    int32_t j_base) {
+#pragma unroll 1
+  for (int32_t st = 0; st < 16; ++st) {
+    const int32_t i0 = i_base + (st >> 2) * 8, j0 = j_base + (st & 3) * 8;
+    if (i0 < f.g.ny && j0 < f.g.nx) batch_replay_pixel<T, FMA, BUF>(f, pass, delta, i0, j0);
+  }
+}
+
+// 64-bit value of lane 0 (the first active lane when the whole wave runs),
+// as a wave-uniform scalar.
+__device__ __forceinline__ int64_t lane0_value(uint64_t x) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(x));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(x >> 32));
+  return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+// Persistent kernel: every wave is an independent worker.  A wave pops one
+// work item (a 32 x 32 region of one pass of one field), waits if needed for
+// the field's previous pass, computes the region's pixels as 16 squares of
+// 8 x 8 lanes, then counts the item done.  There is no block barrier in the
+// loop, so waves of one block never wait for each other, and every branch
+// that steers the loop is wave-uniform.
+template <typename T, bool FMA, bool BRANCHLESS, bool BUF>
+__global__ __launch_bounds__(256) void lic_batch_persistent(const BatchField<T>* fields,
+                                                            const Segment* segs, int32_t nsegs,
+                                                            int64_t total_items,
+                                                            unsigned long long* queue,
+                                                            uint32_t* done, uint32_t* failed,
+                                                            int64_t wait_ticks) {
+  __shared__ uint32_t delta_s[4][16];
+  const int32_t lane = threadIdx.x & 63;
+  uint32_t* delta = delta_s[threadIdx.x >> 6];
+  const int32_t py = lane >> 3, px = lane & 7;
+  for (;;) {
+    uint64_t got = 0;
+    if (lane == 0) got = atomicAdd(queue, 1ull);
+    const int64_t item = lane0_value(got);
+    if (item >= total_items) break;  // the queue is drained
+    int32_t lo = 0, hi = nsegs - 1;
+    while (lo < hi) {
+      const int32_t mid = (lo + hi + 1) >> 1;
+      if (segs[mid].item0 <= item) lo = mid; else hi = mid - 1;
+    }
+    const Segment seg = segs[lo];
+    const BatchField<T>& f = fields[seg.field];
+    if (seg.prev >= 0) {
+      // wait for every item of this field's previous pass (queued earlier,
+      // hence held by running waves whose own waits point further back)
+      const int64_t t0 = wall_clock64();
+      bool stuck = false;
+      for (;;) {
+        const uint32_t d = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(&done[seg.prev], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        if (d >= static_cast<uint32_t>(f.regions)) break;
+        if (wall_clock64() - t0 > wait_ticks) {
+          stuck = true;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(4);
+      }
+      if (stuck) {
+        if (lane == 0) atomicOr(failed, 1u);
+        break;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // see the previous pass's stores
+      __builtin_amdgcn_wave_barrier();
+      if (lane < 16) delta[lane] = move_delta(lane, f.g, BUF ? sizeof(T) : 1u);
+      __builtin_amdgcn_wave_barrier();
+    }
+    const int64_t local = item - seg.item0;
+    const int32_t ry = static_cast<int32_t>(local / f.regions_x);
+    const int32_t rx = static_cast<int32_t>(local - static_cast<int64_t>(ry) * f.regions_x);
+    const int32_t i_base = ry * kRegion + py, j_base = rx * kRegion + px;
+    if (seg.pass == 0) {
+      // one specialised step loop per (uv_mode, recording) pair, each with
+      // its own loop over the region (the branches are uniform)
+      const bool pol = f.uv_mode == 1, rec = f.iterations > 1;
+      if (pol && rec) batch_trace_region<T, FMA, BRANCHLESS, 1, 1, BUF>(f, i_base, j_base);
+      else if (pol) batch_trace_region<T, FMA, BRANCHLESS, 1, 0, BUF>(f, i_base, j_base);
+      else if (rec) batch_trace_region<T, FMA, BRANCHLESS, 0, 1, BUF>(f, i_base, j_base);
+      else batch_trace_region<T, FMA, BRANCHLESS, 0, 0, BUF>(f, i_base, j_base);
+    } else {
+      batch_replay_region<T, FMA, BUF>(f, seg.pass, delta, i_base, j_base);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // publish the wave's stores
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) __hip_atomic_fetch_add(&done[lo], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+template <typename T>
+static hipError_t launch_persistent(int profile, bool buf, const BatchField<T>* fields,
+                                    const Segment* segs, int32_t nsegs, int64_t total_items,
+                                    unsigned long long* queue, uint32_t* done, uint32_t* failed,
+                                    int64_t wait_ticks, int grid, hipStream_t stream) {
+#define RLIC_BATCH_LAUNCH(FMA, BL, BUF)                                                          \
+  hipLaunchKernelGGL((lic_batch_persistent<T, FMA, BL, BUF>), dim3(grid), dim3(256), 0, stream, \
+                     fields, segs, nsegs, total_items, queue, done, failed, wait_ticks)
+  const bool fma = profile & 1, bl = profile & 2;
+  if (buf) {
+    if (fma && bl) RLIC_BATCH_LAUNCH(true, true, true);
+    else if (fma) RLIC_BATCH_LAUNCH(true, false, true);
+    else if (bl) RLIC_BATCH_LAUNCH(false, true, true);
+    else RLIC_BATCH_LAUNCH(false, false, true);
+  } else {
+    if (fma && bl) RLIC_BATCH_LAUNCH(true, true, false);
+    else if (fma) RLIC_BATCH_LAUNCH(true, false, false);
+    else if (bl) RLIC_BATCH_LAUNCH(false, true, false);
+    else RLIC_BATCH_LAUNCH(false, false, false);
+  }
+#undef RLIC_BATCH_LAUNCH
+  return hipGetLastError();
+}
+
+template <typename T>
+static int persistent_grid(int profile, bool buf) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 1024;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    cus = 256;
+  const void* fn = nullptr;
+  const bool fma = profile & 1, bl = profile & 2;
+  if (buf) {
+    fn = fma && bl ? reinterpret_cast<const void*>(&lic_batch_persistent<T, true, true, true>)
+         : fma     ? reinterpret_cast<const void*>(&lic_batch_persistent<T, true, false, true>)
+         : bl      ? reinterpret_cast<const void*>(&lic_batch_persistent<T, false, true, true>)
+                   : reinterpret_cast<const void*>(&lic_batch_persistent<T, false, false, true>);
+  } else {
+    fn = fma && bl ? reinterpret_cast<const void*>(&lic_batch_persistent<T, true, true, false>)
+         : fma     ? reinterpret_cast<const void*>(&lic_batch_persistent<T, true, false, false>)
+         : bl      ? reinterpret_cast<const void*>(&lic_batch_persistent<T, false, true, false>)
+                   : reinterpret_cast<const void*>(&lic_batch_persistent<T, false, false, false>);
+  }
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0) != hipSuccess ||
+      per_cu < 1) {
+    (void)hipGetLastError();
+    per_cu = 4;
+  }
+  return cus * per_cu;
+}
+
+static size_t align_up(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
+
+template <typename T, typename FieldSpec>
+int batch_run(const FieldSpec* specs, int64_t nfields, int profile, hipStream_t stream,
+              BatchScratch& scratch, std::string& err) {
+  // ---- layout of the scratch block
+  std::vector<BatchField<T>> fields(nfields);
+  std::vector<size_t> off_kpad(nfields), off_cells(nfields), off_tmp(nfields), off_codes(nfields);
+  size_t bytes = 0;
+  int64_t pack_blocks = 0;
+  bool buf = true;
+  for (int64_t k = 0; k < nfields; ++k) {
+    const FieldSpec& s = specs[k];
+    const int64_t n = s.ny * s.nx;
+    off_kpad[k] = bytes;
+    bytes = align_up(bytes + sizeof(T) * (s.kernel_len + 2 * kKernelPad));
+    off_cells[k] = bytes;
+    if (s.iterations > 0) bytes = align_up(bytes + sizeof(Cell<T>) * n);
+    off_tmp[k] = bytes;
+    if (s.iterations > 1) bytes = align_up(bytes + sizeof(T) * n);
+    off_codes[k] = bytes;
+    if (s.iterations > 1)
+      bytes = align_up(bytes + sizeof(uint4) * code_chunks(static_cast<int32_t>(s.kernel_len)) * n);
+    if (n * static_cast<int64_t>(sizeof(Cell<T>)) > 0xFFFFFFFFll) buf = false;
+  }
+  // ---- queue segments (pass-major)
+  std::vector<Segment> segs;
+  std::vector<int32_t> last_seg(nfields, -1);
+  int64_t max_it = 0;
+  for (int64_t k = 0; k < nfields; ++k) max_it = std::max<int64_t>(max_it, specs[k].iterations);
+  std::vector<int32_t> regions(nfields), regions_x(nfields);
+  for (int64_t k = 0; k < nfields; ++k) {
+    regions_x[k] = static_cast<int32_t>((specs[k].nx + kRegion - 1) / kRegion);
+    regions[k] = regions_x[k] * static_cast<int32_t>((specs[k].ny + kRegion - 1) / kRegion);
+  }
+  int64_t items = 0;
+  for (int64_t pass = 0; pass < max_it; ++pass) {
+    for (int64_t k = 0; k < nfields; ++k) {
+      if (specs[k].iterations <= pass || specs[k].ny * specs[k].nx == 0) continue;
+      Segment sgm;
+      sgm.item0 = items;
+      sgm.field = static_cast<int32_t>(k);
+      sgm.pass = static_cast<int32_t>(pass);
+      sgm.prev = last_seg[k];
+      sgm.pad = 0;
+      last_seg[k] = static_cast<int32_t>(segs.size());
+      segs.push_back(sgm);
+      items += regions[k];
+    }
+  }
+  const size_t off_fields = bytes;
+  bytes = align_up(bytes + sizeof(BatchField<T>) * nfields);
+  const size_t off_kptrs = bytes;
+  bytes = align_up(bytes + sizeof(const T*) * nfields);
+  const size_t off_segs = bytes;
+  bytes = align_up(bytes + sizeof(Segment) * std::max<size_t>(segs.size(), 1));
+  const size_t off_done = bytes;  // done counters, queue head, failure flag
+  const size_t n_counters = segs.size() + 4;
+  bytes = align_up(bytes + sizeof(uint32_t) * n_counters);
+
+  char* base = nullptr;
+  if (hipError_t e = scratch.reserve(bytes, stream); e != hipSuccess) {
+    err = std::string("allocating batch scratch: ") + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? RLIC_ERR_OUT_OF_MEMORY : RLIC_ERR_HIP;
+  }
+  base = static_cast<char*>(scratch.ptr);
+
+  std::vector<const T*> kptrs(nfields);
+  for (int64_t k = 0; k < nfields; ++k) {
+    const FieldSpec& s = specs[k];
+    BatchField<T>& f = fields[k];
+    f.tex = s.texture; f.tsy = s.tex_sy; f.tsx = s.tex_sx;
+    f.u = s.u; f.usy = s.u_sy; f.usx = s.u_sx;
+    f.v = s.v; f.vsy = s.v_sy; f.vsx = s.v_sx;
+    f.kernel = reinterpret_cast<const T*>(base + off_kpad[k]) + kKernelPad;
+    f.out = s.out;
+    f.tmp = s.iterations > 1 ? reinterpret_cast<T*>(base + off_tmp[k]) : nullptr;
+    f.cells = s.iterations > 0 ? reinterpret_cast<Cell<T>*>(base + off_cells[k]) : nullptr;
+    f.codes = s.iterations > 1 ? reinterpret_cast<uint4*>(base + off_codes[k]) : nullptr;
+    f.g = FieldGeom{static_cast<int32_t>(s.ny), static_cast<int32_t>(s.nx), s.x_left, s.x_right,
+                    s.y_left, s.y_right};
+    f.klen = static_cast<int32_t>(s.kernel_len);
+    f.uv_mode = s.uv_mode;
+    f.iterations = static_cast<int32_t>(s.iterations);
+    f.regions_x = regions_x[k];
+    f.regions = regions[k];
+    f.pack_block0 = pack_blocks;
+    pack_blocks += ((s.nx + 63) / 64) * ((s.ny + 3) / 4);
+    kptrs[k] = s.kernel;
+  }
+  // descriptors go up in one copy from a staging block of this host
+  std::vector<char> host(bytes - off_fields);
+  std::memcpy(host.data(), fields.data(), sizeof(BatchField<T>) * nfields);
+  std::memcpy(host.data() + (off_kptrs - off_fields), kptrs.data(), sizeof(const T*) * nfields);
+  if (!segs.empty())
+    std::memcpy(host.data() + (off_segs - off_fields), segs.data(), sizeof(Segment) * segs.size());
+  hipError_t e = hipMemcpyAsync(base + off_fields, host.data(), off_done - off_fields,
+                                hipMemcpyHostToDevice, stream);
+  if (e != hipSuccess) { err = "uploading batch descriptors"; return RLIC_ERR_HIP; }
+  auto* d_fields = reinterpret_cast<const BatchField<T>*>(base + off_fields);
+  auto* d_kptrs = reinterpret_cast<const T* const*>(base + off_kptrs);
+  auto* d_segs = reinterpret_cast<const Segment*>(base + off_segs);
+  auto* d_done = reinterpret_cast<uint32_t*>(base + off_done);
+  auto* d_queue = reinterpret_cast<unsigned long long*>(d_done + segs.size() + (segs.size() & 1));
+  auto* d_failed = reinterpret_cast<uint32_t*>(d_queue + 1);
+
+  hipLaunchKernelGGL(lic_zero_u32, dim3(std::max<int64_t>(1, std::min<int64_t>(
+                                        (n_counters + 255) / 256, 1024))),
+                     dim3(256), 0, stream, d_done, static_cast<int64_t>(n_counters));
+  hipLaunchKernelGGL(lic_pad_batch<T>, dim3(nfields), dim3(256), 0, stream, d_fields, d_kptrs);
+  if (pack_blocks > 0) {
+    hipLaunchKernelGGL(lic_pack_batch<T>, dim3(pack_blocks), dim3(256), 0, stream, d_fields,
+                       static_cast<int32_t>(nfields));
+  }
+  if ((e = hipGetLastError()) != hipSuccess) {
+    err = std::string("batch set-up kernels: ") + hipGetErrorString(e);
+    return RLIC_ERR_HIP;
+  }
+  if (items > 0) {
+    // four waves (workers) per block; a wait is declared stuck after
+    // RLIC_BATCH_WAIT_MS milliseconds (default 2000) of the device's wall clock
+    const int grid = static_cast<int>(std::min<int64_t>((items + 3) / 4,
+                                                        persistent_grid<T>(profile, buf)));
+    int dev = 0, khz = 100000;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) {
+      (void)hipGetLastError();
+      khz = 100000;
+    }
+    const char* env = std::getenv("RLIC_BATCH_WAIT_MS");
+    const int64_t wait_ms = env ? std::max<int64_t>(0, std::atoll(env)) : 2000;
+    e = launch_persistent<T>(profile, buf, d_fields, d_segs, static_cast<int32_t>(segs.size()),
+                             items, d_queue, d_done, d_failed, wait_ms * khz, grid, stream);
+    if (e != hipSuccess) {
+      err = std::string("batch kernel: ") + hipGetErrorString(e);
+      return RLIC_ERR_HIP;
+    }
+  }
+  uint32_t failed = 0;
+  e = hipMemcpyAsync(&failed, d_failed, sizeof(failed), hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) {
+    err = std::string("batch completion: ") + hipGetErrorString(e);
+    return RLIC_ERR_HIP;
+  }
+  if (failed) {
+    err = "batch: a work item waited too long for the previous pass of its field";
+    return RLIC_ERR_HIP;
+  }
+  return RLIC_OK;
+}
+
+template int batch_run<float, rlic_field_f32>(const rlic_field_f32*, int64_t, int, hipStream_t,
+                                              BatchScratch&, std::string&);
+template int batch_run<double, rlic_field_f64>(const rlic_field_f64*, int64_t, int, hipStream_t,
+                                               BatchScratch&, std::string&);
+
+}  // namespace rlic

@@ -12,6 +12,7 @@
 //   lic_repack_texture  texture slot refresh when passes are re-traced
 #include "lic_device.h"
 #include "lic_launch.h"
+#include "lic_steps.h"
 
 namespace rlic {
 
@@ -62,126 +63,6 @@ __global__ __launch_bounds__(256) void lic_copy2d(const T* __restrict__ src, int
   dst[static_cast<int64_t>(i) * nx + j] = src[i * sy + j * sx];
 }
 
-// Kernel weights are read through a copy padded with kKernelPad zeros on
-// each side, so a group of 8 taps can be loaded without bounds checks.
-constexpr int32_t kKernelPad = 8;
-
-// Move-code layout: a direction's steps are cut into chunks of 32 steps
-// (4 words of 8 four-bit codes); chunk c of pixel p (row-major index
-// i * nx + j) is the 16-byte vector codes4[(c0 + c) * ny * nx + p], so one
-// lane moves 32 steps of codes with one load or store, and the tracing and
-// replay kernels may tile the image differently.
-constexpr int32_t kChunkSteps = 32;
-
-__host__ __device__ inline int32_t chunks_of(int32_t nsteps) {
-  return (nsteps + kChunkSteps - 1) / kChunkSteps;
-}
-
-// One streamline direction of one pixel (src/lib.rs:304-362), starting from
-// the vector (pu, pv) of the start cell, already loaded.  The position is
-// (row, j) with row = i * nx, both in the gather's units.  Returns the
-// updated accumulator.  With REC, writes the move code of every step.
-//
-// The step is written without divergent branches: a lane whose line has
-// ended (NaN vector) keeps executing with its position frozen and its
-// accumulator untouched, and a lane with a zero vector keeps its position
-// (src/lib.rs:242-244); steps run in groups of 8 (one code word, one batch
-// of 8 weights) and a wave leaves the loop once all its lines have ended.
-template <typename T, bool FMA, bool BRANCHLESS, bool POL, bool FORWARD, bool REC, bool BUF>
-__device__ __forceinline__ T trace_direction(T acc, T pu, T pv, uint32_t row, uint32_t j,
-                                             const CellGather<T, BUF>& cell,
-                                             const T* __restrict__ kernel, int32_t kmid,
-                                             int32_t nsteps, const StepGeom& sg,
-                                             uint4* __restrict__ codes, int64_t np, int64_t p,
-                                             int32_t c0) {
-  T fx = T(0.5), fy = T(0.5);      // src/lib.rs:315-318
-  T last_u = T(0), last_v = T(0);  // src/lib.rs:320-323
-  bool alive = true;
-  const uint32_t unit = cell.unit;
-  // select operands: edge positions, edge targets (src/lib.rs:83-95: a
-  // closed side keeps the edge cell, a periodic side wraps to the opposite
-  // edge cell) and row steps, held in VGPRs across the loop
-  const uint32_t nxm1 = lane_copy(sg.nxm1), last_row = lane_copy(sg.last_row);
-  const uint32_t x_hi = lane_copy(sg.x_hi_per ? 0u : sg.nxm1);
-  const uint32_t x_lo = lane_copy(sg.x_lo_per ? sg.nxm1 : 0u);
-  const uint32_t y_hi = lane_copy(sg.y_hi_per ? 0u : sg.last_row);
-  const uint32_t y_lo = lane_copy(sg.y_lo_per ? sg.last_row : 0u);
-  const uint32_t row_up = lane_copy(sg.nx), row_down = lane_copy(0u - sg.nx);
-  const int32_t nchunks = chunks_of(nsteps);
-  for (int32_t ch = 0; ch < nchunks; ++ch) {
-    uint32_t words[4] = {0u, 0u, 0u, 0u};
-    bool any_alive = true;
-#pragma unroll 1
-    for (int g = 0; g < 4; ++g) {
-      const int32_t s0 = ch * kChunkSteps + 8 * g;
-      if (s0 >= nsteps || !any_alive) break;  // uniform
-      T kw[8];
-#pragma unroll
-      for (int q = 0; q < 8; ++q)
-        kw[q] = kernel[FORWARD ? kmid + 1 + s0 + q : kmid - 1 - s0 - q];
-      uint32_t word = 0;
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        if (s0 + q >= nsteps) break;  // uniform: last group only
-        // a NaN component ends the line (src/lib.rs:336-338)
-        const bool ends = __builtin_isunordered(pu, pv);
-        const bool live = alive && !ends;
-        if constexpr (POL) {  // src/lib.rs:339-347, unfused dot product
-          const T d1 = pu * last_u;
-          const T d2 = pv * last_v;
-          const bool flip = d1 + d2 < T(0);
-          pu = flip ? -pu : pu;
-          pv = flip ? -pv : pv;
-          last_u = pu;
-          last_v = pv;
-        }
-        const T mu = FORWARD ? pu : -pu;  // src/lib.rs:348-351
-        const T mv = FORWARD ? pv : -pv;
-        const T tx = time_to_next_pixel<T, FMA, BRANCHLESS>(mu, fx);
-        const T ty = time_to_next_pixel<T, FMA, BRANCHLESS>(mv, fy);
-        // x edge first when tx < ty; ties and NaN go to y (src/lib.rs:249-269)
-        const bool xs = tx < ty;
-        // no move when both components are +-0 (src/lib.rs:242-244)
-        const bool moves = live && (mu != T(0) || mv != T(0));
-        // update_state (src/lib.rs:208-233) for either axis, then keep one
-        const bool pos_x = mu >= T(0);
-        const bool pos_y = mv >= T(0);
-        const T fy_after_x = macc<FMA>(tx, mv, fy);
-        const T fx_after_y = macc<FMA>(ty, mu, fx);
-        const T fx_new = xs ? (pos_x ? T(0) : T(1)) : fx_after_y;
-        const T fy_new = xs ? fy_after_x : (pos_y ? T(0) : T(1));
-        fx = moves ? fx_new : fx;
-        fy = moves ? fy_new : fy;
-        // the crossed coordinate with the boundary rule (src/lib.rs:83-95)
-        const bool edge_x = j == (pos_x ? nxm1 : 0u);
-        const uint32_t j_to = edge_x ? (pos_x ? x_hi : x_lo) : j + (pos_x ? unit : 0u - unit);
-        const bool edge_y = row == (pos_y ? last_row : 0u);
-        const uint32_t row_to = edge_y ? (pos_y ? y_hi : y_lo) : row + (pos_y ? row_up : row_down);
-        j = (moves && xs) ? j_to : j;
-        row = (moves && !xs) ? row_to : row;
-        const CellVal<T> cv = cell(row + j);
-        const T next_acc = macc<FMA>(kw[q], cv.x, acc);  // src/lib.rs:353-360
-        acc = live ? next_acc : acc;
-        pu = cv.u;
-        pv = cv.v;
-        if constexpr (REC) {
-          const uint32_t code_x = (pos_x ? kMoveXPlus : kMoveXMinus) | (edge_x ? kMoveEdge : 0u);
-          const uint32_t code_y = (pos_y ? kMoveYPlus : kMoveYMinus) | (edge_y ? kMoveEdge : 0u);
-          uint32_t code = moves ? (xs ? code_x : code_y) : kMoveStay;
-          code = (alive && ends) ? kMoveStop : code;
-          word |= code << (4 * q);
-        }
-        alive = live;
-      }
-      words[g] = word;
-      any_alive = __builtin_amdgcn_ballot_w64(alive) != 0;
-    }
-    if constexpr (REC) codes[(c0 + ch) * np + p] = make_uint4(words[0], words[1], words[2], words[3]);
-    if (!any_alive) break;  // every line of the wave ended
-  }
-  return acc;
-}
-
 template <typename T, bool FMA, bool BRANCHLESS, bool POL, bool REC, bool BUF>
 __global__ __launch_bounds__(256) void lic_trace(const Cell<T>* __restrict__ cells,
                                                  const T* __restrict__ kernel, int32_t klen,
@@ -208,62 +89,6 @@ __global__ __launch_bounds__(256) void lic_trace(const Cell<T>* __restrict__ cel
       acc, c0.u, c0.v, row0, jj0, cell, kernel, kmid, n_bwd, sg, codes, np, p,
       chunks_of(n_fwd));
   out[static_cast<int64_t>(i0) * g.nx + j0] = acc;
-}
-
-// Replay of one direction from its move codes.  Chunks of 32 steps: the
-// chunk after the current one is loaded before the current one is used, and
-// within a chunk each group of 8 steps decodes its 8 positions through the
-// delta table, issues the 8 texture gathers together and accumulates them
-// in step order.  Codes are valid up to the step where the line ended (the
-// tracing pass wrote the stop code there), so a lane stops reading codes
-// once its line has ended and its later codes are taken as "stay".
-template <typename T, bool FMA, bool FORWARD, bool BUF>
-__device__ __forceinline__ T replay_direction(T acc, uint32_t pos,
-                                              const ValueGather<T, BUF>& x,
-                                              const T* __restrict__ kernel, int32_t kmid,
-                                              int32_t nsteps, const uint32_t* delta,
-                                              const uint4* __restrict__ codes, int64_t np,
-                                              int64_t p, int32_t c0) {
-  const int32_t nchunks = chunks_of(nsteps);
-  bool alive = true;
-  uint4 next = nchunks > 0 ? codes[c0 * np + p] : make_uint4(0u, 0u, 0u, 0u);
-  for (int32_t ch = 0; ch < nchunks; ++ch) {
-    const uint4 cur = next;
-    if (ch + 1 < nchunks && alive) next = codes[(c0 + ch + 1) * np + p];
-    const uint32_t words[4] = {cur.x, cur.y, cur.z, cur.w};
-#pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      const int32_t s0 = ch * kChunkSteps + 8 * g;
-      if (s0 >= nsteps) break;  // uniform
-      const uint32_t word = alive ? words[g] : 0u;  // 0 = stay
-      // nibble 0xF marks the step where the line ended
-      const uint32_t stops = word & (word >> 1) & (word >> 2) & (word >> 3) & 0x11111111u;
-      const int32_t valid = alive ? (stops ? (__builtin_ctz(stops) >> 2) : 8) : 0;
-      T kw[8];
-#pragma unroll
-      for (int q = 0; q < 8; ++q)
-        kw[q] = kernel[FORWARD ? kmid + 1 + s0 + q : kmid - 1 - s0 - q];
-      uint32_t at[8];
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        pos += delta[(word >> (4 * q)) & 15u];
-        at[q] = pos;
-      }
-      T xv[8];
-#pragma unroll
-      for (int q = 0; q < 8; ++q) xv[q] = x(at[q]);
-      const int32_t lim = nsteps - s0;
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        if (q >= lim) break;  // uniform: last group only
-        const T next_acc = macc<FMA>(kw[q], xv[q], acc);
-        acc = q < valid ? next_acc : acc;
-      }
-      alive = valid == 8;
-    }
-    if (__builtin_amdgcn_ballot_w64(alive) == 0) break;
-  }
-  return acc;
 }
 
 template <typename T, bool FMA, bool BUF>

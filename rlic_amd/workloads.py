@@ -112,6 +112,80 @@ def polarization_field(n: int = 16384, dtype=np.float32, seed: int = 3) -> Workl
                     "closed", "polarization", 1)
 
 
+def batch_uneven(n_fields: int = 512, seed: int = 4, device=None, dtype=np.float32,
+                 max_side: int = 2048, min_side: int = 128) -> list[dict]:
+    """The uneven batch: n_fields independent convolve calls.
+
+    Sides log-uniform in [min_side, max_side] (redrawn until the aspect
+    ratio is within 1:4..4:1), kernel lengths uniform over odd 17..257
+    (triangular or boxcar), each axis periodic or closed, uv_mode velocity or
+    polarization, iterations uniform in 1..10, u and v sums of 8 Fourier
+    modes, texture U[0,1).  The parameters come from a NumPy generator seeded
+    with ``seed``; with ``device`` the arrays are generated there with torch
+    (same distributions, different samples), otherwise with NumPy.
+    Returns a list of dicts with keys texture, u, v, kernel, uv_mode,
+    boundaries, iterations.
+    """
+    rng = np.random.default_rng(seed)
+    fields = []
+    for k in range(n_fields):
+        while True:
+            h, w = np.exp(rng.uniform(np.log(min_side), np.log(max_side), size=2)).round()
+            if 0.25 <= h / w <= 4.0:
+                break
+        h, w = int(h), int(w)
+        klen = int(rng.choice(np.arange(17, 258, 2)))
+        if rng.random() < 0.5:
+            kernel = 1 - np.abs(np.linspace(-1, 1, klen))
+        else:
+            kernel = np.ones(klen)
+        bx = "periodic" if rng.random() < 0.5 else "closed"
+        by = "periodic" if rng.random() < 0.5 else "closed"
+        mode = "polarization" if rng.random() < 0.5 else "velocity"
+        iterations = int(rng.integers(1, 11))
+        fseed = int(rng.integers(0, 2**31))
+        if device is not None:
+            import torch
+
+            gen = torch.Generator(device=device).manual_seed(fseed)
+            tdt = torch.float32 if np.dtype(dtype) == np.float32 else torch.float64
+            tex = torch.rand((h, w), generator=gen, device=device, dtype=torch.float64).to(tdt)
+            u = _torch_modes(gen, h, w, device).to(tdt)
+            v = _torch_modes(gen, h, w, device).to(tdt)
+            kern = torch.from_numpy(kernel).to(device=device, dtype=tdt)
+        else:
+            frng = np.random.default_rng(fseed)
+            tex = frng.random((h, w)).astype(dtype)
+            u = fourier_modes(frng, (h, w)).astype(dtype)
+            v = fourier_modes(frng, (h, w)).astype(dtype)
+            kern = kernel.astype(dtype)
+        fields.append(dict(texture=tex, u=u, v=v, kernel=kern, uv_mode=mode,
+                           boundaries={"x": bx, "y": by}, iterations=iterations))
+    return fields
+
+
+def _torch_modes(gen, h: int, w: int, device, n_modes: int = 8, kmax: int = 16):
+    import torch
+
+    jj = torch.arange(w, device=device, dtype=torch.float64) / w
+    ii = torch.arange(h, device=device, dtype=torch.float64) / h
+    out = torch.zeros((h, w), device=device, dtype=torch.float64)
+    for _ in range(n_modes):
+        kx, ky = torch.randint(1, kmax + 1, (2,), generator=gen, device=device).tolist()
+        a = torch.randn((), generator=gen, device=device, dtype=torch.float64)
+        phi = torch.rand((), generator=gen, device=device, dtype=torch.float64) * 2 * np.pi
+        out += a * torch.cos(2 * np.pi * (kx * jj[None, :] + ky * ii[:, None]) + phi)
+    return out
+
+
+def pixel_steps(fields: list[dict]) -> int:
+    total = 0
+    for f in fields:
+        h, w = f["texture"].shape
+        total += int(h) * int(w) * (int(f["kernel"].shape[0]) - 1) * int(f["iterations"])
+    return total
+
+
 WORKLOADS = {
     "readme": readme_example,
     "plain4096": plain_at_scale,

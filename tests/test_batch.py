@@ -1,0 +1,119 @@
+"""Batched convolution: one persistent launch for many uneven fields gives,
+field by field, the same bits as the single calls (GPU), and rejects bad
+fields before any work (CPU)."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import rlic_amd
+from oracle import cpu
+from rlic_amd import _native
+from rlic_amd.batch import Field, convolve_batch
+
+
+def _mixed_fields(rng, n):
+    fields = []
+    for k in range(n):
+        dtype = np.float32 if k % 3 else np.float64
+        ny, nx = int(rng.integers(1, 90)), int(rng.integers(1, 140))
+        klen = int(rng.integers(1, 70))
+        tex = rng.random((ny, nx)).astype(dtype)
+        u = (rng.random((ny, nx)) - 0.5).astype(dtype)
+        v = (rng.random((ny, nx)) - 0.5).astype(dtype)
+        if k % 4 == 1:
+            u[rng.random((ny, nx)) < 0.03] = np.nan
+            v[:, nx // 2] = 0.0
+        if k % 5 == 2:
+            u = np.asfortranarray(u)  # strided input
+        fields.append(Field(
+            tex, u, v, rng.random(klen).astype(dtype),
+            uv_mode="polarization" if k % 2 else "velocity",
+            boundaries={"x": ("periodic", "closed")[k % 2], "y": ("closed", "periodic")[(k // 2) % 2]},
+            iterations=int(rng.integers(0, 5)),
+        ))
+    return fields
+
+
+@pytest.mark.gpu
+def test_batch_matches_single_calls():
+    rng = np.random.default_rng(42)
+    fields = _mixed_fields(rng, 40)
+    outs = convolve_batch(fields)
+    for k, (f, out) in enumerate(zip(fields, outs)):
+        single = rlic_amd.convolve(f.texture, f.u, f.v, kernel=f.kernel, uv_mode=f.uv_mode,
+                                   boundaries=f.boundaries, iterations=f.iterations)
+        assert out.dtype == single.dtype and out.shape == single.shape, k
+        assert np.array_equal(out, single, equal_nan=True), f"field {k} differs"
+
+
+@pytest.mark.gpu
+def test_batch_matches_oracle_on_device_tensors(gpu_device):
+    import torch
+
+    rng = np.random.default_rng(7)
+    fields = _mixed_fields(rng, 12)
+    dev_fields = [Field(*(torch.from_numpy(np.ascontiguousarray(a)).to(gpu_device)
+                          for a in (f.texture, f.u, f.v, f.kernel)),
+                        uv_mode=f.uv_mode, boundaries=f.boundaries, iterations=f.iterations)
+                  for f in fields]
+    outs = convolve_batch(dev_fields)
+    torch.cuda.synchronize()
+    for k, (f, out) in enumerate(zip(fields, outs)):
+        if f.iterations == 0:
+            exp = f.texture
+        else:
+            exp = cpu.convolve(np.ascontiguousarray(f.texture), np.ascontiguousarray(f.u),
+                               np.ascontiguousarray(f.v), kernel=f.kernel, uv_mode=f.uv_mode,
+                               boundaries=f.boundaries, iterations=f.iterations)
+        assert np.array_equal(out.cpu().numpy(), exp, equal_nan=True), f"field {k}"
+
+
+@pytest.mark.gpu
+def test_batch_uneven_workload_sample():
+    """A few fields of the uneven-batch distribution, checked against single calls."""
+    import torch
+
+    from rlic_amd.workloads import batch_uneven
+
+    dev = torch.device("cuda:0")
+    fields = batch_uneven(16, seed=11, device=dev, max_side=512)
+    outs = convolve_batch(fields)
+    for k, (f, out) in enumerate(zip(fields, outs)):
+        single = rlic_amd.convolve(f["texture"], f["u"], f["v"], kernel=f["kernel"],
+                                   uv_mode=f["uv_mode"], boundaries=f["boundaries"],
+                                   iterations=f["iterations"])
+        assert torch.equal(out, single), f"field {k}"
+
+
+def test_batch_validation_runs_first():
+    good = Field(np.ones((4, 4)), np.ones((4, 4)), np.ones((4, 4)), np.ones(3))
+    bad = Field(np.ones((4, 4)), np.ones((4, 5)), np.ones((4, 4)), np.ones(3))
+    with pytest.raises(ValueError, match="^Shape mismatch"):
+        convolve_batch([good, bad])
+    assert convolve_batch([]) == []
+
+
+def test_batch_abi_rejects_bad_fields_without_a_gpu():
+    lib = _native.load()
+    arr = (_native.FieldF64 * 2)()
+    a = np.ones((4, 4))
+    import ctypes
+
+    p = ctypes.cast(a.ctypes.data, ctypes.POINTER(ctypes.c_double))
+    for slot in arr:
+        slot.texture = slot.u = slot.v = slot.out = slot.kernel = p
+        slot.ny = slot.nx = 4
+        slot.tex_sy = slot.u_sy = slot.v_sy = 4
+        slot.tex_sx = slot.u_sx = slot.v_sx = 1
+        slot.kernel_len = 3
+        slot.iterations = 1
+    arr[1].uv_mode = 9
+    assert lib.rlic_convolve_batch_f64(arr, 2, 3, None) < 0
+    assert _native.last_error().startswith("field 1: unknown uv_mode")
+    arr[1].uv_mode = 0
+    arr[0].kernel_len = 0
+    assert lib.rlic_convolve_batch_f64(arr, 2, 3, None) < 0
+    assert "kernel is empty" in _native.last_error()
+    assert lib.rlic_convolve_batch_f64(arr, 0, 3, None) == 0
