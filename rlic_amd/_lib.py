@@ -149,11 +149,14 @@ def convolve(
     return native(texture, (u, v, uv_mode), kernel, (bs.x, bs.y), iterations)
 
 
-def validate(texture, u, v, *, kernel, uv_mode, boundaries, iterations):
+def validate(texture, u, v, *, kernel, uv_mode, boundaries, iterations, check_values=True):
     """Run every input check of ``convolve`` (src/rlic/_lib.py:135-205).
 
     Raises the single problem found, or all of them as an ExceptionGroup;
     returns the normalised BoundarySet and the common dtype otherwise.
+    ``check_values=False`` skips the two checks that read array elements
+    (negative texture values, non-finite kernel weights), for callers that
+    run those checks themselves.
     """
     problems: list[Exception] = []
     if iterations < 0:
@@ -187,7 +190,7 @@ def validate(texture, u, v, *, kernel, uv_mode, boundaries, iterations):
         problems.append(
             ValueError(f"Expected a texture with exactly two dimensions. Got texture.ndim={texture.ndim}")
         )
-    if _any_negative(texture):
+    if check_values and _any_negative(texture):
         problems.append(
             ValueError("Found invalid texture element(s). Expected only positive values.")
         )
@@ -203,7 +206,7 @@ def validate(texture, u, v, *, kernel, uv_mode, boundaries, iterations):
         problems.append(
             ValueError(f"Expected a kernel with exactly one dimension. Got kernel.ndim={kernel.ndim}")
         )
-    if _any_non_finite(kernel):
+    if check_values and _any_non_finite(kernel):
         problems.append(ValueError("Found non-finite value(s) in kernel."))
 
     bs = BoundarySet.from_spec(boundaries)

@@ -12,6 +12,8 @@ import ctypes
 import os
 from pathlib import Path
 
+import numpy as np
+
 LIB_NAME = "librlic_hip.so"
 LIB_PATH = Path(__file__).resolve().parent / LIB_NAME
 
@@ -51,6 +53,23 @@ def _field_struct(name: str, ctype) -> type:
 
 FieldF32 = _field_struct("FieldF32", ctypes.c_float)
 FieldF64 = _field_struct("FieldF64", ctypes.c_double)
+
+# The same layout as a NumPy structured dtype (pointers as addresses), for
+# filling many descriptors at once.
+FIELD_DTYPE = np.dtype([
+    ("texture", "<u8"), ("tex_sy", "<i8"), ("tex_sx", "<i8"),
+    ("u", "<u8"), ("u_sy", "<i8"), ("u_sx", "<i8"),
+    ("v", "<u8"), ("v_sy", "<i8"), ("v_sx", "<i8"),
+    ("ny", "<i8"), ("nx", "<i8"),
+    ("kernel", "<u8"), ("kernel_len", "<i8"),
+    ("uv_mode", "<i4"), ("x_left", "<i4"), ("x_right", "<i4"), ("y_left", "<i4"),
+    ("y_right", "<i4"),
+    ("iterations", "<i8"),
+    ("out", "<u8"),
+], align=True)
+assert FIELD_DTYPE.itemsize == ctypes.sizeof(FieldF32) == ctypes.sizeof(FieldF64)
+assert all(FIELD_DTYPE.fields[name][1] == getattr(FieldF32, name).offset
+           for name, _ in FieldF32._fields_)
 
 SIGNATURES: dict[str, tuple[list, object]] = {
     "rlic_convolve_batch_f32": ([ctypes.POINTER(FieldF32), _i64, _int, _vp], _int),

@@ -52,8 +52,41 @@ def _as_field(f) -> Field:
     return Field(texture, u, v, **(f[3] if len(f) > 3 else {}))
 
 
+def _first_value_problem(items: list[Field], host: bool) -> int | None:
+    """Index of the first field with a negative texture element or a
+    non-finite kernel weight, or None.  Device fields are checked with one
+    reduction per dtype over all fields and a single host synchronisation."""
+    if host:
+        for k, f in enumerate(items):
+            if np.any(f.texture < 0) or not np.all(np.isfinite(f.kernel)):
+                return k
+        return None
+    if not items:
+        return None
+    import torch
+
+    bad = torch.zeros((), dtype=torch.bool, device=items[0].texture.device)
+    for dt in dict.fromkeys(f.texture.dtype for f in items):
+        texs = [f.texture.reshape(-1) for f in items if f.texture.dtype == dt]
+        bad |= (torch.cat(texs) < 0).any()
+    for dt in dict.fromkeys(f.kernel.dtype for f in items):
+        kerns = [f.kernel.reshape(-1) for f in items if f.kernel.dtype == dt]
+        bad |= (~torch.isfinite(torch.cat(kerns))).any()
+    if not bool(bad):
+        return None
+    for k, f in enumerate(items):  # error path: find the field
+        if bool((f.texture < 0).any()) or bool((~torch.isfinite(f.kernel)).any()):
+            return k
+    return None
+
+
 def convolve_batch(fields: Sequence, *, profile: int | str | None = None, device=None) -> list:
-    """Convolve every field; returns the outputs in order."""
+    """Convolve every field; returns the outputs in order.
+
+    Every field is validated as ``convolve`` validates its arguments before
+    anything runs; the first invalid field raises what ``convolve`` would
+    raise for it.
+    """
     import torch
 
     items = [_as_field(f) for f in fields]
@@ -63,12 +96,26 @@ def convolve_batch(fields: Sequence, *, profile: int | str | None = None, device
         _core.ARITHMETIC_PROFILES[profile] if isinstance(profile, str) else int(profile))
     host = not _core._is_torch(items[0].texture)
     checked = []
-    for f in items:  # every field is validated before anything touches the device
+    first_bad = None
+    for idx, f in enumerate(items):  # structure first, element values below
         arrays = (f.texture, f.u, f.v, f.kernel)
         if any(_core._is_torch(a) for a in arrays) == host:
             raise TypeError("a batch holds NumPy arrays or torch tensors, not both")
-        checked.append(validate(f.texture, f.u, f.v, kernel=f.kernel, uv_mode=f.uv_mode,
-                                boundaries=f.boundaries, iterations=f.iterations))
+        try:
+            checked.append(validate(f.texture, f.u, f.v, kernel=f.kernel, uv_mode=f.uv_mode,
+                                    boundaries=f.boundaries, iterations=f.iterations,
+                                    check_values=False))
+        except Exception:
+            first_bad = idx
+            break
+    value_bad = _first_value_problem(items[:first_bad] if first_bad is not None else items, host)
+    if value_bad is not None:
+        first_bad = value_bad
+    if first_bad is not None:
+        f = items[first_bad]
+        validate(f.texture, f.u, f.v, kernel=f.kernel, uv_mode=f.uv_mode,
+                 boundaries=f.boundaries, iterations=f.iterations)
+        raise AssertionError("unreachable: the field failed validation above")
     dev = torch.device(device) if device is not None else (
         torch.device("cuda", torch.cuda.current_device()) if host else items[0].texture.device)
 
@@ -95,27 +142,20 @@ def convolve_batch(fields: Sequence, *, profile: int | str | None = None, device
             group = [p for p in prepared if p[1] == dtype]
             if not group:
                 continue
-            arr = (struct * len(group))()
-            keep = []
-            ctype = ctypes.c_float if dtype == np.float32 else ctypes.c_double
-            ptr = lambda t: ctypes.cast(ctypes.c_void_p(t.data_ptr()),  # noqa: E731
-                                        ctypes.POINTER(ctype))
-            for slot, (idx, _, (tex, u, v, k), bs, mode, iters) in zip(arr, group):
+            rows = []
+            for idx, _, (tex, u, v, k), bs, mode, iters in group:
                 ny, nx = tex.shape
                 out = torch.empty((ny, nx), dtype=tdtype, device=dev)
-                keep.append(out)
                 outputs[idx] = out
-                slot.texture, slot.tex_sy, slot.tex_sx = ptr(tex), tex.stride(0), tex.stride(1)
-                slot.u, slot.u_sy, slot.u_sx = ptr(u), u.stride(0), u.stride(1)
-                slot.v, slot.v_sy, slot.v_sx = ptr(v), v.stride(0), v.stride(1)
-                slot.ny, slot.nx = ny, nx
-                slot.kernel, slot.kernel_len = ptr(k), k.shape[0]
-                slot.uv_mode = mode
-                slot.x_left, slot.x_right = _BOUND[bs.x[0]], _BOUND[bs.x[1]]
-                slot.y_left, slot.y_right = _BOUND[bs.y[0]], _BOUND[bs.y[1]]
-                slot.iterations = iters
-                slot.out = ptr(out)
-            status = entry(arr, len(group), prof, ctypes.c_void_p(stream))
+                rows.append((tex.data_ptr(), tex.stride(0), tex.stride(1),
+                             u.data_ptr(), u.stride(0), u.stride(1),
+                             v.data_ptr(), v.stride(0), v.stride(1),
+                             ny, nx, k.data_ptr(), k.shape[0], mode,
+                             _BOUND[bs.x[0]], _BOUND[bs.x[1]], _BOUND[bs.y[0]], _BOUND[bs.y[1]],
+                             iters, out.data_ptr()))
+            arr = np.array(rows, dtype=_native.FIELD_DTYPE)
+            status = entry(arr.ctypes.data_as(ctypes.POINTER(struct)), len(group), prof,
+                           ctypes.c_void_p(stream))
             _native.raise_for_status(status)
     if host:
         torch.cuda.synchronize(dev)

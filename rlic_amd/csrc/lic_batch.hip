@@ -27,6 +27,8 @@
 namespace rlic {
 
 constexpr int32_t kRegion = 32;  // pixels per side of a work item (one wave)
+constexpr int32_t kPendSlots = 8;        // segments a wave holds unpublished counts for
+constexpr uint32_t kPublishItems = 32;  // completed items per release of a wave
 
 template <typename T>
 struct BatchField {
@@ -37,6 +39,7 @@ struct BatchField {
   const T* v;
   int64_t vsy, vsx;
   const T* kernel;  // first weight of the zero-padded copy
+  int64_t kernel_off;  // the same weight as an element offset into the weight pool
   T* out;           // last pass writes here
   T* tmp;           // ping-pong partner (passes > 1)
   Cell<T>* cells;
@@ -110,7 +113,9 @@ __global__ void lic_zero_u32(uint32_t* p, int64_t n) {
 }
 
 template <typename T, bool FMA, bool BRANCHLESS, int POL, int REC, bool BUF>
-__device__ __forceinline__ void batch_trace_pixel(const BatchField<T>& f, int32_t i0, int32_t j0) {
+__device__ __forceinline__ void batch_trace_pixel(const BatchField<T>& f,
+                                                  const T* __restrict__ kernel, int32_t i0,
+                                                  int32_t j0) {
   const CellGather<T, BUF> cell(f.cells, static_cast<int64_t>(f.g.ny) * f.g.nx);
   const StepGeom sg = make_step_geom(f.g, cell.unit);
   const int32_t kmid = f.klen / 2;
@@ -120,28 +125,29 @@ __device__ __forceinline__ void batch_trace_pixel(const BatchField<T>& f, int32_
   const uint32_t row0 = static_cast<uint32_t>(i0) * sg.nx;
   const uint32_t jj0 = static_cast<uint32_t>(j0) * cell.unit;
   const CellVal<T> c0 = cell(row0 + jj0);
-  T acc = macc<FMA>(f.kernel[kmid], c0.x, T(0));  // src/lib.rs:375-383
+  T acc = macc<FMA>(kernel[kmid], c0.x, T(0));  // src/lib.rs:375-383
   acc = trace_direction<T, FMA, BRANCHLESS, POL, true, REC, BUF>(
-      acc, c0.u, c0.v, row0, jj0, cell, f.kernel, kmid, n_fwd, sg, f.codes, np, p, 0);
+      acc, c0.u, c0.v, row0, jj0, cell, kernel, kmid, n_fwd, sg, f.codes, np, p, 0);
   acc = trace_direction<T, FMA, BRANCHLESS, POL, false, REC, BUF>(
-      acc, c0.u, c0.v, row0, jj0, cell, f.kernel, kmid, n_bwd, sg, f.codes, np, p,
+      acc, c0.u, c0.v, row0, jj0, cell, kernel, kmid, n_bwd, sg, f.codes, np, p,
       chunks_of(n_fwd));
   pass_buffer(f, 0)[p] = acc;
 }
 
 template <typename T, bool FMA, bool BUF>
-__device__ __forceinline__ void batch_replay_pixel(const BatchField<T>& f, int32_t pass, const uint32_t* delta,
-                                   int32_t i0, int32_t j0) {
+__device__ __forceinline__ void batch_replay_pixel(const BatchField<T>& f,
+                                                   const T* __restrict__ kernel, int32_t pass,
+                                                   const uint32_t* delta, int32_t i0, int32_t j0) {
   const ValueGather<T, BUF> x(pass_buffer(f, pass - 1), static_cast<int64_t>(f.g.ny) * f.g.nx);
   const int32_t kmid = f.klen / 2;
   const int32_t n_fwd = f.klen - 1 - kmid, n_bwd = kmid;
   const int64_t np = static_cast<int64_t>(f.g.ny) * f.g.nx;
   const int64_t idx0 = static_cast<int64_t>(i0) * f.g.nx + j0;
   const uint32_t pos0 = static_cast<uint32_t>(idx0) * x.unit;
-  T acc = macc<FMA>(f.kernel[kmid], x(pos0), T(0));
-  acc = replay_direction<T, FMA, true, BUF>(acc, pos0, x, f.kernel, kmid, n_fwd, delta, f.codes,
+  T acc = macc<FMA>(kernel[kmid], x(pos0), T(0));
+  acc = replay_direction<T, FMA, true, BUF>(acc, pos0, x, kernel, kmid, n_fwd, delta, f.codes,
                                             np, idx0, 0);
-  acc = replay_direction<T, FMA, false, BUF>(acc, pos0, x, f.kernel, kmid, n_bwd, delta, f.codes,
+  acc = replay_direction<T, FMA, false, BUF>(acc, pos0, x, kernel, kmid, n_bwd, delta, f.codes,
                                              np, idx0, chunks_of(n_fwd));
   pass_buffer(f, pass)[idx0] = acc;
 }
@@ -149,23 +155,26 @@ __device__ __forceinline__ void batch_replay_pixel(const BatchField<T>& f, int32
 // The 32 x 32 region of a work item as 16 squares of 8 x 8 lanes; (i_base,
 // j_base) is the lane's pixel in the first square.
 template <typename T, bool FMA, bool BRANCHLESS, int POL, int REC, bool BUF>
-__device__ __forceinline__ void batch_trace_region(const BatchField<T>& f, int32_t i_base,
+__device__ __forceinline__ void batch_trace_region(const BatchField<T>& f,
+                                                   const T* __restrict__ kernel, int32_t i_base,
                                                    int32_t j_base) {
 #pragma unroll 1
   for (int32_t st = 0; st < 16; ++st) {
     const int32_t i0 = i_base + (st >> 2) * 8, j0 = j_base + (st & 3) * 8;
-    if (i0 < f.g.ny && j0 < f.g.nx) batch_trace_pixel<T, FMA, BRANCHLESS, POL, REC, BUF>(f, i0, j0);
+    if (i0 < f.g.ny && j0 < f.g.nx)
+      batch_trace_pixel<T, FMA, BRANCHLESS, POL, REC, BUF>(f, kernel, i0, j0);
   }
 }
 
 template <typename T, bool FMA, bool BUF>
-__device__ __forceinline__ void batch_replay_region(const BatchField<T>& f, int32_t pass,
+__device__ __forceinline__ void batch_replay_region(const BatchField<T>& f,
+                                                    const T* __restrict__ kernel, int32_t pass,
                                                     const uint32_t* delta, int32_t i_base,
                                                     int32_t j_base) {
 #pragma unroll 1
   for (int32_t st = 0; st < 16; ++st) {
     const int32_t i0 = i_base + (st >> 2) * 8, j0 = j_base + (st & 3) * 8;
-    if (i0 < f.g.ny && j0 < f.g.nx) batch_replay_pixel<T, FMA, BUF>(f, pass, delta, i0, j0);
+    if (i0 < f.g.ny && j0 < f.g.nx) batch_replay_pixel<T, FMA, BUF>(f, kernel, pass, delta, i0, j0);
   }
 }
 
@@ -184,8 +193,10 @@ __device__ __forceinline__ int64_t lane0_value(uint64_t x) {
 // loop, so waves of one block never wait for each other, and every branch
 // that steers the loop is wave-uniform.
 template <typename T, bool FMA, bool BRANCHLESS, bool BUF>
-__global__ __launch_bounds__(256) void lic_batch_persistent(const BatchField<T>* fields,
-                                                            const Segment* segs, int32_t nsegs,
+__global__ __launch_bounds__(256) void lic_batch_persistent(const BatchField<T>* __restrict__ fields,
+                                                            const Segment* __restrict__ segs,
+                                                            const T* __restrict__ kpool,
+                                                            int32_t nsegs,
                                                             int64_t total_items,
                                                             unsigned long long* queue,
                                                             uint32_t* done, uint32_t* failed,
@@ -194,6 +205,54 @@ __global__ __launch_bounds__(256) void lic_batch_persistent(const BatchField<T>*
   const int32_t lane = threadIdx.x & 63;
   uint32_t* delta = delta_s[threadIdx.x >> 6];
   const int32_t py = lane >> 3, px = lane & 7;
+  // Completed items are published in batches: per-segment counts of up to
+  // kPendSlots segments are held by the wave and published together behind
+  // one release fence (the fence writes back L2, too costly per item) once
+  // kPublishItems items are pending, when a new segment finds no free slot,
+  // before the wave waits (so no wave waits while it holds unpublished
+  // work) and when the queue is drained.
+  int32_t pseg[kPendSlots] = {};
+  uint32_t pcnt[kPendSlots] = {};
+  int32_t npend = 0;
+  uint32_t pitems = 0;
+  int32_t fenced_prev = -1;  // segment whose completion the last acquire covered
+  auto publish = [&]() {
+    if (npend != 0) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __builtin_amdgcn_wave_barrier();
+      if (lane == 0) {
+#pragma unroll
+        for (int32_t k = 0; k < kPendSlots; ++k)
+          if (k < npend)
+            __hip_atomic_fetch_add(&done[pseg[k]], pcnt[k], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+      }
+      npend = 0;
+      pitems = 0;
+    }
+  };
+  auto note_done = [&](int32_t sgi) {
+    bool found = false;
+#pragma unroll
+    for (int32_t k = 0; k < kPendSlots; ++k) {
+      if (k < npend && pseg[k] == sgi) {
+        pcnt[k] += 1u;
+        found = true;
+      }
+    }
+    if (!found) {
+      if (npend == kPendSlots) publish();
+#pragma unroll
+      for (int32_t k = 0; k < kPendSlots; ++k) {
+        if (k == npend) {
+          pseg[k] = sgi;
+          pcnt[k] = 1u;
+        }
+      }
+      ++npend;
+    }
+    if (++pitems >= kPublishItems) publish();
+  };
   for (;;) {
     uint64_t got = 0;
     if (lane == 0) got = atomicAdd(queue, 1ull);
@@ -207,25 +266,35 @@ __global__ __launch_bounds__(256) void lic_batch_persistent(const BatchField<T>*
     const Segment seg = segs[lo];
     const BatchField<T>& f = fields[seg.field];
     if (seg.prev >= 0) {
-      // wait for every item of this field's previous pass (queued earlier,
-      // hence held by running waves whose own waits point further back)
-      const int64_t t0 = wall_clock64();
-      bool stuck = false;
-      for (;;) {
-        const uint32_t d = __builtin_amdgcn_readfirstlane(
-            __hip_atomic_load(&done[seg.prev], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        if (d >= static_cast<uint32_t>(f.regions)) break;
-        if (wall_clock64() - t0 > wait_ticks) {
-          stuck = true;
+      // every item of this field's previous pass must be complete; those
+      // items were queued earlier, hence popped by running waves whose own
+      // waits point further back
+      auto complete = [&]() {
+        return __builtin_amdgcn_readfirstlane(__hip_atomic_load(
+                   &done[seg.prev], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >=
+               static_cast<uint32_t>(f.regions);
+      };
+      if (!complete()) {
+        publish();
+        const int64_t t0 = wall_clock64();
+        bool stuck = false;
+        while (!complete()) {
+          if (wall_clock64() - t0 > wait_ticks) {
+            stuck = true;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(4);
+        }
+        if (stuck) {
+          if (lane == 0) atomicOr(failed, 1u);
           break;
         }
-        __builtin_amdgcn_s_sleep(4);
+        fenced_prev = -1;  // a completion newer than the last acquire
       }
-      if (stuck) {
-        if (lane == 0) atomicOr(failed, 1u);
-        break;
+      if (seg.prev != fenced_prev) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // see the previous pass's stores
+        fenced_prev = seg.prev;
       }
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // see the previous pass's stores
       __builtin_amdgcn_wave_barrier();
       if (lane < 16) delta[lane] = move_delta(lane, f.g, BUF ? sizeof(T) : 1u);
       __builtin_amdgcn_wave_barrier();
@@ -234,21 +303,21 @@ __global__ __launch_bounds__(256) void lic_batch_persistent(const BatchField<T>*
     const int32_t ry = static_cast<int32_t>(local / f.regions_x);
     const int32_t rx = static_cast<int32_t>(local - static_cast<int64_t>(ry) * f.regions_x);
     const int32_t i_base = ry * kRegion + py, j_base = rx * kRegion + px;
+    const T* __restrict__ kern = kpool + f.kernel_off;  // uniform: weights via scalar loads
     if (seg.pass == 0) {
       // one specialised step loop per (uv_mode, recording) pair, each with
       // its own loop over the region (the branches are uniform)
       const bool pol = f.uv_mode == 1, rec = f.iterations > 1;
-      if (pol && rec) batch_trace_region<T, FMA, BRANCHLESS, 1, 1, BUF>(f, i_base, j_base);
-      else if (pol) batch_trace_region<T, FMA, BRANCHLESS, 1, 0, BUF>(f, i_base, j_base);
-      else if (rec) batch_trace_region<T, FMA, BRANCHLESS, 0, 1, BUF>(f, i_base, j_base);
-      else batch_trace_region<T, FMA, BRANCHLESS, 0, 0, BUF>(f, i_base, j_base);
+      if (pol && rec) batch_trace_region<T, FMA, BRANCHLESS, 1, 1, BUF>(f, kern, i_base, j_base);
+      else if (pol) batch_trace_region<T, FMA, BRANCHLESS, 1, 0, BUF>(f, kern, i_base, j_base);
+      else if (rec) batch_trace_region<T, FMA, BRANCHLESS, 0, 1, BUF>(f, kern, i_base, j_base);
+      else batch_trace_region<T, FMA, BRANCHLESS, 0, 0, BUF>(f, kern, i_base, j_base);
     } else {
-      batch_replay_region<T, FMA, BUF>(f, seg.pass, delta, i_base, j_base);
+      batch_replay_region<T, FMA, BUF>(f, kern, seg.pass, delta, i_base, j_base);
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // publish the wave's stores
-    __builtin_amdgcn_wave_barrier();
-    if (lane == 0) __hip_atomic_fetch_add(&done[lo], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    note_done(lo);
   }
+  publish();
 }
 
 // ---------------------------------------------------------------------------
@@ -257,12 +326,13 @@ __global__ __launch_bounds__(256) void lic_batch_persistent(const BatchField<T>*
 
 template <typename T>
 static hipError_t launch_persistent(int profile, bool buf, const BatchField<T>* fields,
-                                    const Segment* segs, int32_t nsegs, int64_t total_items,
+                                    const Segment* segs, const T* kpool, int32_t nsegs,
+                                    int64_t total_items,
                                     unsigned long long* queue, uint32_t* done, uint32_t* failed,
                                     int64_t wait_ticks, int grid, hipStream_t stream) {
 #define RLIC_BATCH_LAUNCH(FMA, BL, BUF)                                                          \
   hipLaunchKernelGGL((lic_batch_persistent<T, FMA, BL, BUF>), dim3(grid), dim3(256), 0, stream, \
-                     fields, segs, nsegs, total_items, queue, done, failed, wait_ticks)
+                     fields, segs, kpool, nsegs, total_items, queue, done, failed, wait_ticks)
   const bool fma = profile & 1, bl = profile & 2;
   if (buf) {
     if (fma && bl) RLIC_BATCH_LAUNCH(true, true, true);
@@ -381,6 +451,7 @@ int batch_run(const FieldSpec* specs, int64_t nfields, int profile, hipStream_t 
     f.u = s.u; f.usy = s.u_sy; f.usx = s.u_sx;
     f.v = s.v; f.vsy = s.v_sy; f.vsx = s.v_sx;
     f.kernel = reinterpret_cast<const T*>(base + off_kpad[k]) + kKernelPad;
+    f.kernel_off = static_cast<int64_t>(off_kpad[k] / sizeof(T)) + kKernelPad;
     f.out = s.out;
     f.tmp = s.iterations > 1 ? reinterpret_cast<T*>(base + off_tmp[k]) : nullptr;
     f.cells = s.iterations > 0 ? reinterpret_cast<Cell<T>*>(base + off_cells[k]) : nullptr;
@@ -437,7 +508,8 @@ int batch_run(const FieldSpec* specs, int64_t nfields, int profile, hipStream_t 
     }
     const char* env = std::getenv("RLIC_BATCH_WAIT_MS");
     const int64_t wait_ms = env ? std::max<int64_t>(0, std::atoll(env)) : 2000;
-    e = launch_persistent<T>(profile, buf, d_fields, d_segs, static_cast<int32_t>(segs.size()),
+    e = launch_persistent<T>(profile, buf, d_fields, d_segs, reinterpret_cast<const T*>(base),
+                             static_cast<int32_t>(segs.size()),
                              items, d_queue, d_done, d_failed, wait_ms * khz, grid, stream);
     if (e != hipSuccess) {
       err = std::string("batch kernel: ") + hipGetErrorString(e);

@@ -117,3 +117,40 @@ def test_batch_abi_rejects_bad_fields_without_a_gpu():
     assert lib.rlic_convolve_batch_f64(arr, 2, 3, None) < 0
     assert "kernel is empty" in _native.last_error()
     assert lib.rlic_convolve_batch_f64(arr, 0, 3, None) == 0
+
+
+def test_batch_raises_for_the_first_invalid_field():
+    ok = Field(np.ones((4, 4)), np.ones((4, 4)), np.ones((4, 4)), np.ones(3))
+    neg = Field(-np.ones((4, 4)), np.ones((4, 4)), np.ones((4, 4)), np.ones(3))
+    nan_kernel = Field(np.ones((4, 4)), np.ones((4, 4)), np.ones((4, 4)), np.array([1.0, np.nan]))
+    shape = Field(np.ones((4, 4)), np.ones((4, 5)), np.ones((4, 4)), np.ones(3))
+    with pytest.raises(ValueError, match="^Found invalid texture element"):
+        convolve_batch([ok, neg, shape])
+    with pytest.raises(ValueError, match="^Shape mismatch"):
+        convolve_batch([ok, shape, neg])
+    with pytest.raises(ValueError, match="^Found non-finite value"):
+        convolve_batch([ok, nan_kernel, neg])
+    # several problems in one field: the same ExceptionGroup as convolve
+    both = Field(-np.ones((4, 4)), np.ones((4, 5)), np.ones((4, 4)), np.ones(3))
+    with pytest.raises(Exception) as info:
+        convolve_batch([ok, both])
+    assert [type(e) for e in info.value.exceptions] == [ValueError, ValueError]
+
+
+@pytest.mark.gpu
+def test_batch_device_validation(gpu_device):
+    import torch
+
+    def t(a):
+        return torch.from_numpy(np.asarray(a, dtype=np.float32)).to(gpu_device)
+
+    ok = Field(t(np.ones((4, 4))), t(np.ones((4, 4))), t(np.ones((4, 4))), t(np.ones(3)))
+    neg = Field(t(-np.eye(4)), t(np.ones((4, 4))), t(np.ones((4, 4))), t(np.ones(3)))
+    inf_kernel = Field(t(np.ones((4, 4))), t(np.ones((4, 4))), t(np.ones((4, 4))),
+                       t([1.0, np.inf, 1.0]))
+    with pytest.raises(ValueError, match="^Found invalid texture element"):
+        convolve_batch([ok, neg])
+    with pytest.raises(ValueError, match="^Found non-finite value"):
+        convolve_batch([ok, inf_kernel, neg])
+    out = convolve_batch([ok, ok])
+    assert all(o.shape == (4, 4) for o in out)
