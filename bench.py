@@ -145,6 +145,111 @@ def bench_batch(args, dev) -> int:
     return 0
 
 
+def bench_serve(args, dev) -> int:
+    """Serving: Poisson arrivals of convolve requests, continuous batching."""
+    import torch
+
+    from rlic_amd.serve import Server
+    from rlic_amd.workloads import serving_pool, serving_request
+
+    budget = 64 << 30
+    log("bench: generating the serving field pool on the device")
+    pool = serving_pool(96, seed=5, device=dev)
+    rng = np.random.default_rng(5)
+
+    def submit(srv, r, arrival=None, keep=False):
+        return srv.submit(r["texture"], r["u"], r["v"], kernel=r["kernel"],
+                          uv_mode=r["uv_mode"], boundaries=r["boundaries"],
+                          iterations=r["iterations"], keep=keep, arrival=arrival)
+
+    # capacity: a burst of requests all arriving at once
+    with Server(budget, device=dev) as srv:
+        for r in [serving_request(rng, pool) for _ in range(8)]:  # warm-up
+            submit(srv, r)
+        srv.run()
+        burst = [serving_request(rng, pool) for _ in range(args.serve_burst)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        reqs = [submit(srv, r) for r in burst]
+        srv.run()
+        torch.cuda.synchronize()
+        cap_s = time.perf_counter() - t0
+    cap_work = sum(q.pixel_steps for q in reqs)
+    capacity = cap_work / cap_s
+    mean_work = cap_work / len(reqs)
+    lam = 0.7 * capacity / mean_work
+    log(f"bench: capacity {capacity / 1e9:.1f} Gpixel-steps/s over {len(reqs)} requests; "
+        f"Poisson rate {lam:.1f} requests/s (70% load)")
+
+    # Poisson arrivals; a quarter of them follow up a finished kept request
+    n = args.serve_requests
+    gaps = rng.exponential(1.0 / lam, size=n)
+    offsets = np.cumsum(gaps)
+    plans = [(serving_request(rng, pool), rng.random() < 0.25, rng.random() < 0.5,
+              int(rng.integers(1, 21))) for _ in range(n)]
+    done: list = []
+    kept_done: list = []
+    with Server(budget, device=dev) as srv:
+        torch.cuda.synchronize()
+        t_start = time.perf_counter()
+        k = 0
+        while k < n or srv.queued or srv.active:
+            now = time.perf_counter() - t_start
+            while k < n and offsets[k] <= now:
+                r, follow, keep, more = plans[k]
+                arrival = t_start + offsets[k]
+                if follow and kept_done:
+                    base = kept_done[int(rng.integers(len(kept_done)))]
+                    kept_done.remove(base)
+                    q = srv.follow_up(base, more, arrival=arrival)
+                else:
+                    q = submit(srv, r, arrival=arrival, keep=keep)
+                if q.state == "done":
+                    done.append(q)
+                k += 1
+            if not (srv.queued or srv.active):
+                if k < n:
+                    time.sleep(max(0.0, offsets[k] - (time.perf_counter() - t_start)))
+                continue
+            for q in srv.step():
+                done.append(q)
+                if q.keep:
+                    kept_done.append(q)
+        t_end = time.perf_counter()
+        steps_run, evictions, readmissions = srv.steps, srv.evictions, srv.readmissions
+    lat = np.array([q.latency for q in done]) * 1e3
+    work = sum(q.pixel_steps for q in done)
+    span = t_end - t_start
+    value = work / span / 1e9
+    line = {
+        "metric": "Gpixel-steps/s (serving, Poisson arrivals at 70% load)",
+        "value": round(value, 3),
+        "unit": "Gpixel-steps/s",
+        "n_gpus": 1,
+        "steps": steps_run,
+        "warmup": 8,
+        "ms_per_step": round(span / max(steps_run, 1) * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "serving_poisson", "requests": n,
+                   "rate_per_s": round(lam, 2), "budget_bytes": budget,
+                   "capacity_gpixel_steps_per_s": round(capacity / 1e9, 3),
+                   "followups": sum(q.base is not None for q in done)},
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
+                       "p99": round(float(np.percentile(lat, 99)), 2),
+                       "max": round(float(lat.max()), 2)},
+        "evictions": evictions,
+        "readmissions": readmissions,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +258,10 @@ def main() -> int:
     ap.add_argument("--workload", default="plain4096")
     ap.add_argument("--cpu-rows", type=int, default=2048,
                     help="rows of pass 1 timed on the CPU oracle (0 = skip)")
+    ap.add_argument("--serve-burst", type=int, default=96,
+                    help="serving: requests of the capacity burst")
+    ap.add_argument("--serve-requests", type=int, default=1500,
+                    help="serving: Poisson-arrival requests")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="N",
                     help="first run the library's copy kernel N times on a 1 GiB array "
                          "(known HBM bytes, for calibrating FETCH_SIZE/WRITE_SIZE counters)")
@@ -185,6 +294,8 @@ def main() -> int:
         torch.cuda.empty_cache()
     if args.workload == "batch512":
         return bench_batch(args, dev)
+    if args.workload == "serve":
+        return bench_serve(args, dev)
     log(f"bench: building workload {args.workload}")
     wl = WORKLOADS[args.workload]()
     fn = _core.convolve_f32 if wl.texture.dtype == np.float32 else _core.convolve_f64

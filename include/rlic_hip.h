@@ -133,6 +133,53 @@ int rlic_convolve_batch_f32(const rlic_field_f32* fields, int64_t n_fields, int 
 int rlic_convolve_batch_f64(const rlic_field_f64* fields, int64_t n_fields, int profile,
                             void* stream);
 
+/* Serving: fields resident in device memory, advanced one pass per step.
+ * No counterpart in the reference (one blocking call per field,
+ * src/lib.rs:450-482); every field's passes follow the semantics of that
+ * call, and its output after n passes equals rlic_convolve_* with
+ * iterations = n, bit for bit, whatever else shares the steps.
+ *
+ * rlic_server_create: a server on the current device that keeps at most
+ *   budget_bytes of field state resident and runs its work on `stream`.
+ *   NULL on failure (see rlic_last_error).
+ * rlic_server_admit_*: copies the kernel weights and packs (u, v, texture)
+ *   (device memory; `out` is ignored), registers the
+ *   field for `iterations` (>= 1) passes and returns its handle.  keep != 0
+ *   keeps the move record so that rlic_server_extend can add passes later.
+ *   RLIC_ERR_OUT_OF_MEMORY when the field does not fit the free budget
+ *   (nothing is admitted then; retry after releasing fields).
+ * rlic_server_field_bytes_*: the budget bytes such a field occupies.
+ * rlic_server_step: runs one pass of every field with passes left and
+ *   writes the handles whose requested passes are now all done into
+ *   finished[0 .. *n_finished) (capacity: at least the number of active
+ *   fields).  Returns after the work has completed.
+ * rlic_server_extend: requests `more` passes of a kept field whose passes
+ *   are all done (a follow-up: it continues from the field's last output).
+ * rlic_server_read_*: copies the field's latest output (ny x nx, C order)
+ *   to `out` (device or host memory); synchronous.
+ * rlic_server_release: frees the field's state and budget.
+ * rlic_server_usage: budget bytes in use, resident and active fields.
+ * Calls on one server are serialized; distinct servers are independent. */
+typedef struct rlic_server rlic_server;
+rlic_server* rlic_server_create(int64_t budget_bytes, int profile, void* stream);
+void rlic_server_destroy(rlic_server* server);
+int rlic_server_admit_f32(rlic_server* server, const rlic_field_f32* field, int keep,
+                          int64_t* handle);
+int rlic_server_admit_f64(rlic_server* server, const rlic_field_f64* field, int keep,
+                          int64_t* handle);
+int64_t rlic_server_field_bytes_f32(int64_t ny, int64_t nx, int64_t kernel_len, int keep,
+                                    int64_t iterations);
+int64_t rlic_server_field_bytes_f64(int64_t ny, int64_t nx, int64_t kernel_len, int keep,
+                                    int64_t iterations);
+int rlic_server_step(rlic_server* server, int64_t* finished, int64_t capacity,
+                     int64_t* n_finished);
+int rlic_server_extend(rlic_server* server, int64_t handle, int64_t more);
+int rlic_server_read_f32(rlic_server* server, int64_t handle, float* out);
+int rlic_server_read_f64(rlic_server* server, int64_t handle, double* out);
+int rlic_server_release(rlic_server* server, int64_t handle);
+int rlic_server_usage(rlic_server* server, int64_t* used_bytes, int64_t* resident,
+                      int64_t* active);
+
 /* Message for the calling thread's most recent failed call ("" if none).
  * No counterpart in the reference, whose errors abort (Cargo.toml:23). */
 const char* rlic_last_error(void);

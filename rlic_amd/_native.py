@@ -71,7 +71,21 @@ assert FIELD_DTYPE.itemsize == ctypes.sizeof(FieldF32) == ctypes.sizeof(FieldF64
 assert all(FIELD_DTYPE.fields[name][1] == getattr(FieldF32, name).offset
            for name, _ in FieldF32._fields_)
 
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
 SIGNATURES: dict[str, tuple[list, object]] = {
+    "rlic_server_create": ([_i64, _int, _vp], _vp),
+    "rlic_server_destroy": ([_vp], None),
+    "rlic_server_admit_f32": ([_vp, ctypes.POINTER(FieldF32), _int, _i64p], _int),
+    "rlic_server_admit_f64": ([_vp, ctypes.POINTER(FieldF64), _int, _i64p], _int),
+    "rlic_server_field_bytes_f32": ([_i64, _i64, _i64, _int, _i64], _i64),
+    "rlic_server_field_bytes_f64": ([_i64, _i64, _i64, _int, _i64], _i64),
+    "rlic_server_step": ([_vp, _i64p, _i64, _i64p], _int),
+    "rlic_server_extend": ([_vp, _i64, _i64], _int),
+    "rlic_server_read_f32": ([_vp, _i64, _vp], _int),
+    "rlic_server_read_f64": ([_vp, _i64, _vp], _int),
+    "rlic_server_release": ([_vp, _i64], _int),
+    "rlic_server_usage": ([_vp, _i64p, _i64p, _i64p], _int),
     "rlic_convolve_batch_f32": ([ctypes.POINTER(FieldF32), _i64, _int, _vp], _int),
     "rlic_convolve_batch_f64": ([ctypes.POINTER(FieldF64), _i64, _int, _vp], _int),
     "rlic_convolve_f32": (_CONVOLVE_ARGTYPES, _int),

@@ -26,6 +26,7 @@
 #include "../../include/rlic_hip.h"
 #include "lic_batch.h"
 #include "lic_launch.h"
+#include "lic_serve.h"
 
 namespace {
 
@@ -396,6 +397,41 @@ int convolve_entry(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t u
   return RLIC_OK;
 }
 
+template <typename T, typename FieldSpec>
+int server_admit(rlic::Server* server, const FieldSpec* f, int keep, int64_t* handle) {
+  t_last_error.clear();
+  if (!server || !f || !handle) return set_error(RLIC_ERR_INVALID_ARGUMENT, "null argument");
+  const int rc = validate<T>(f->texture, f->u, f->v, f->ny, f->nx, f->kernel, f->kernel_len,
+                             f->uv_mode, f->x_left, f->x_right, f->y_left, f->y_right,
+                             f->iterations, f->texture);
+  if (rc != RLIC_OK) return rc;
+  if (f->iterations < 1)
+    return set_error(RLIC_ERR_INVALID_ARGUMENT, "a served field needs at least one pass");
+  std::string err;
+  const bool record = keep != 0 || f->iterations > 1;
+  const int st = server->admit<T>(f->texture, f->tex_sy, f->tex_sx, f->u, f->u_sy, f->u_sx,
+                                  f->v, f->v_sy, f->v_sx, f->ny, f->nx, f->kernel,
+                                  f->kernel_len, f->uv_mode, f->x_left, f->x_right, f->y_left,
+                                  f->y_right, f->iterations, record, handle, err);
+  return st == RLIC_OK ? st : set_error(st, "%s", err.c_str());
+}
+
+template <typename T>
+int64_t server_field_bytes(int64_t ny, int64_t nx, int64_t klen, int keep, int64_t iterations) {
+  if (ny < 0 || nx < 0 || klen < 0) return -1;
+  return static_cast<int64_t>(
+      rlic::Server::field_bytes<T>(ny, nx, klen, keep != 0 || iterations > 1));
+}
+
+template <typename T>
+int server_read(rlic::Server* server, int64_t handle, T* out) {
+  t_last_error.clear();
+  if (!server) return set_error(RLIC_ERR_INVALID_ARGUMENT, "null server");
+  std::string err;
+  const int st = server->read<T>(handle, out, err);
+  return st == RLIC_OK ? st : set_error(st, "%s", err.c_str());
+}
+
 }  // namespace
 
 extern "C" {
@@ -429,6 +465,108 @@ int rlic_convolve_batch_f32(const rlic_field_f32* fields, int64_t n_fields, int 
 int rlic_convolve_batch_f64(const rlic_field_f64* fields, int64_t n_fields, int profile,
                             void* stream) {
   return batch_entry<double, rlic_field_f64>(fields, n_fields, profile, stream);
+}
+
+struct rlic_server {
+  rlic::Server core;
+  rlic_server(int64_t budget, int profile, hipStream_t stream) : core(budget, profile, stream) {}
+};
+
+rlic_server* rlic_server_create(int64_t budget_bytes, int profile, void* stream) {
+  t_last_error.clear();
+  if (budget_bytes < 0) {
+    set_error(RLIC_ERR_INVALID_ARGUMENT, "negative budget");
+    return nullptr;
+  }
+  if (profile < 0 || profile > 3) {
+    set_error(RLIC_ERR_INVALID_ARGUMENT, "unknown arithmetic profile %d", profile);
+    return nullptr;
+  }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    hip_error(hipGetLastError(), "hipGetDevice");
+    return nullptr;
+  }
+  configure_pool(dev);
+  return new rlic_server(budget_bytes, profile, static_cast<hipStream_t>(stream));
+}
+
+void rlic_server_destroy(rlic_server* server) { delete server; }
+
+int rlic_server_admit_f32(rlic_server* server, const rlic_field_f32* field, int keep,
+                          int64_t* handle) {
+  return server_admit<float>(server ? &server->core : nullptr, field, keep, handle);
+}
+
+int rlic_server_admit_f64(rlic_server* server, const rlic_field_f64* field, int keep,
+                          int64_t* handle) {
+  return server_admit<double>(server ? &server->core : nullptr, field, keep, handle);
+}
+
+int64_t rlic_server_field_bytes_f32(int64_t ny, int64_t nx, int64_t kernel_len, int keep,
+                                    int64_t iterations) {
+  return server_field_bytes<float>(ny, nx, kernel_len, keep, iterations);
+}
+
+int64_t rlic_server_field_bytes_f64(int64_t ny, int64_t nx, int64_t kernel_len, int keep,
+                                    int64_t iterations) {
+  return server_field_bytes<double>(ny, nx, kernel_len, keep, iterations);
+}
+
+int rlic_server_step(rlic_server* server, int64_t* finished, int64_t capacity,
+                     int64_t* n_finished) {
+  t_last_error.clear();
+  if (!server || !n_finished || capacity < 0 || (capacity > 0 && !finished))
+    return set_error(RLIC_ERR_INVALID_ARGUMENT, "invalid argument");
+  int64_t active = 0;
+  server->core.usage(nullptr, nullptr, &active);
+  if (capacity < active)
+    return set_error(RLIC_ERR_INVALID_ARGUMENT,
+                     "capacity %lld is below the %lld active fields",
+                     static_cast<long long>(capacity), static_cast<long long>(active));
+  std::vector<int64_t> done;
+  std::string err;
+  int st;
+  {
+    LaunchTimer timer("lic_serve_step", server->core.stream());
+    st = server->core.step(done, err);
+  }
+  if (st != RLIC_OK) return set_error(st, "%s", err.c_str());
+  std::copy(done.begin(), done.end(), finished);
+  *n_finished = static_cast<int64_t>(done.size());
+  return RLIC_OK;
+}
+
+int rlic_server_extend(rlic_server* server, int64_t handle, int64_t more) {
+  t_last_error.clear();
+  if (!server || more < 0) return set_error(RLIC_ERR_INVALID_ARGUMENT, "invalid argument");
+  std::string err;
+  const int st = server->core.extend(handle, more, err);
+  return st == RLIC_OK ? st : set_error(st, "%s", err.c_str());
+}
+
+int rlic_server_read_f32(rlic_server* server, int64_t handle, float* out) {
+  return server_read<float>(server ? &server->core : nullptr, handle, out);
+}
+
+int rlic_server_read_f64(rlic_server* server, int64_t handle, double* out) {
+  return server_read<double>(server ? &server->core : nullptr, handle, out);
+}
+
+int rlic_server_release(rlic_server* server, int64_t handle) {
+  t_last_error.clear();
+  if (!server) return set_error(RLIC_ERR_INVALID_ARGUMENT, "null server");
+  std::string err;
+  const int st = server->core.release(handle, err);
+  return st == RLIC_OK ? st : set_error(st, "%s", err.c_str());
+}
+
+int rlic_server_usage(rlic_server* server, int64_t* used_bytes, int64_t* resident,
+                      int64_t* active) {
+  t_last_error.clear();
+  if (!server) return set_error(RLIC_ERR_INVALID_ARGUMENT, "null server");
+  server->core.usage(used_bytes, resident, active);
+  return RLIC_OK;
 }
 
 const char* rlic_last_error(void) { return t_last_error.c_str(); }

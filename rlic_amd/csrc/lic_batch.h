@@ -1,16 +1,19 @@
-// Batched convolution (lic_batch.hip): host entry used by the C ABI.
+// Work-queue launches over many fields (lic_batch.hip), shared by the
+// batched call and the serving step.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/rlic_hip.h"
+#include "lic_device.h"
 
 namespace rlic {
 
-// Stream-ordered scratch block owned by one batch call.
+// Stream-ordered device scratch owned by one call.
 struct BatchScratch {
   void* ptr = nullptr;
   hipStream_t stream = nullptr;
@@ -25,6 +28,52 @@ struct BatchScratch {
     return hipMallocAsync(&ptr, bytes ? bytes : 1, s);
   }
 };
+
+// Pixels per side of one work item (a square region of one field).
+constexpr int32_t kRegion = 32;
+
+// One field of a queue launch, as the device reads it.
+template <typename T>
+struct BatchField {
+  const T* tex;  // texture, u, v with their strides: read by the pack step only
+  int64_t tsy, tsx;
+  const T* u;
+  int64_t usy, usx;
+  const T* v;
+  int64_t vsy, vsx;
+  const T* kernel;      // set by launch_queue: first weight of the padded copy
+  int64_t kernel_off;   // set by launch_queue: that weight as an offset into the pool
+  T* out;               // pack step: receives the texture when iterations == 0
+  Cell<T>* cells;       // packed (u, v, texture)
+  uint4* codes;         // move record, or nullptr
+  FieldGeom g;
+  int32_t klen, uv_mode;
+  int32_t iterations;   // pack step only (0: copy the texture to out)
+  int32_t regions_x, regions;  // set by launch_queue
+  int64_t pack_block0;         // set by launch_queue
+};
+
+// A run of queue items: every region of one pass of one field.
+struct Segment {
+  int64_t item0;    // set by launch_queue
+  int32_t field;
+  int32_t pass;     // 0 traces from the cells, > 0 replays the move record
+  int32_t prev;     // segment whose items must all be done first (-1: none)
+  int32_t record;   // pass 0: write the move record
+  const void* src;  // replay input: the previous pass's output
+  void* dst;        // this pass's output
+};
+
+// Pads every field's kernel weights (kernel_src[k], device memory) into a
+// weight pool, packs every field's (texture, u, v) into its cells when
+// `pack`, then runs the items of every segment in one persistent launch.
+// With `deps`, the items of a segment wait for segment `prev`, which must
+// come earlier in `segs`; without, no segment may have a `prev`.  Fills the
+// fields' and segments' launch_queue members and synchronizes `stream`.
+template <typename T>
+int launch_queue(std::vector<BatchField<T>>& fields, const std::vector<const T*>& kernel_src,
+                 std::vector<Segment>& segs, bool deps, bool pack, int profile,
+                 hipStream_t stream, std::string& err);
 
 // Runs every field of the batch (already validated) in one persistent
 // launch; synchronizes `stream` before returning.

@@ -1,16 +1,20 @@
-// Batched convolution of many independent fields in one persistent launch.
+// Many fields in one persistent launch from a device work queue.
 //
 // Each field is one `rlic.convolve` call (src/lib.rs:408-443) with its own
 // size, kernel, boundaries, uv_mode and iteration count.  Work items are
-// 32 x 32 pixel regions of one pass of one field, queued pass-major: every
-// field's pass 1, then every field's pass 2, and so on.  The waves of a
-// persistent grid pop items from one atomic counter.  A pass-p item of a
-// field waits until the field's pass p-1 is complete; the items it waits for
-// were queued earlier, so they were popped by waves that are already running
-// and whose own waits point further back, hence every wait ends (a wait longer than a
-// time limit still ends, and makes the call fail).  Pass 1 traces (and
-// records moves when more passes follow), later passes replay the moves
-// with gathers through L2.
+// 32 x 32 pixel regions of one pass of one field; a segment is the run of
+// items of one (field, pass).  The waves of a persistent grid pop items from
+// one atomic counter.  Pass 0 traces the streamlines from the packed cells
+// (and records the moves when more passes follow); later passes replay the
+// moves with gathers through L2.
+//
+// The batched call queues every pass of every field in one launch,
+// pass-major (every field's pass 0, then every field's pass 1, ...), and a
+// pass-p item waits until the field's pass p-1 is complete.  The items it
+// waits for were queued earlier, so they were popped by waves that are
+// already running and whose own waits point further back: every wait ends
+// (a wait longer than a time limit still ends, and makes the call fail).
+// A serving step queues one pass of each active field, with no waits.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -26,42 +30,8 @@
 
 namespace rlic {
 
-constexpr int32_t kRegion = 32;  // pixels per side of a work item (one wave)
-constexpr int32_t kPendSlots = 8;        // segments a wave holds unpublished counts for
+constexpr int32_t kPendSlots = 8;       // segments a wave holds unpublished counts for
 constexpr uint32_t kPublishItems = 32;  // completed items per release of a wave
-
-template <typename T>
-struct BatchField {
-  const T* tex;
-  int64_t tsy, tsx;
-  const T* u;
-  int64_t usy, usx;
-  const T* v;
-  int64_t vsy, vsx;
-  const T* kernel;  // first weight of the zero-padded copy
-  int64_t kernel_off;  // the same weight as an element offset into the weight pool
-  T* out;           // last pass writes here
-  T* tmp;           // ping-pong partner (passes > 1)
-  Cell<T>* cells;
-  uint4* codes;     // move record (passes > 1)
-  FieldGeom g;
-  int32_t klen, uv_mode, iterations;
-  int32_t regions_x, regions;
-  int64_t pack_block0;  // first block of this field in the batched pack grid
-};
-
-struct Segment {
-  int64_t item0;   // first queue item of this (pass, field)
-  int32_t field, pass;
-  int32_t prev;    // segment of the same field's previous pass, -1 for pass 1
-  int32_t pad;
-};
-
-template <typename T>
-__device__ __forceinline__ T* pass_buffer(const BatchField<T>& f, int32_t pass) {
-  // the last pass lands in `out`; passes alternate backwards from there
-  return ((f.iterations - 1 - pass) % 2 == 0) ? f.out : f.tmp;
-}
 
 // Kernel weights of every field into its zero-padded slot (one block per field).
 template <typename T>
@@ -114,8 +84,8 @@ __global__ void lic_zero_u32(uint32_t* p, int64_t n) {
 
 template <typename T, bool FMA, bool BRANCHLESS, int POL, int REC, bool BUF>
 __device__ __forceinline__ void batch_trace_pixel(const BatchField<T>& f,
-                                                  const T* __restrict__ kernel, int32_t i0,
-                                                  int32_t j0) {
+                                                  const T* __restrict__ kernel, T* dst,
+                                                  int32_t i0, int32_t j0) {
   const CellGather<T, BUF> cell(f.cells, static_cast<int64_t>(f.g.ny) * f.g.nx);
   const StepGeom sg = make_step_geom(f.g, cell.unit);
   const int32_t kmid = f.klen / 2;
@@ -131,14 +101,15 @@ __device__ __forceinline__ void batch_trace_pixel(const BatchField<T>& f,
   acc = trace_direction<T, FMA, BRANCHLESS, POL, false, REC, BUF>(
       acc, c0.u, c0.v, row0, jj0, cell, kernel, kmid, n_bwd, sg, f.codes, np, p,
       chunks_of(n_fwd));
-  pass_buffer(f, 0)[p] = acc;
+  dst[p] = acc;
 }
 
 template <typename T, bool FMA, bool BUF>
 __device__ __forceinline__ void batch_replay_pixel(const BatchField<T>& f,
-                                                   const T* __restrict__ kernel, int32_t pass,
-                                                   const uint32_t* delta, int32_t i0, int32_t j0) {
-  const ValueGather<T, BUF> x(pass_buffer(f, pass - 1), static_cast<int64_t>(f.g.ny) * f.g.nx);
+                                                   const T* __restrict__ kernel, const T* src,
+                                                   T* dst, const uint32_t* delta, int32_t i0,
+                                                   int32_t j0) {
+  const ValueGather<T, BUF> x(src, static_cast<int64_t>(f.g.ny) * f.g.nx);
   const int32_t kmid = f.klen / 2;
   const int32_t n_fwd = f.klen - 1 - kmid, n_bwd = kmid;
   const int64_t np = static_cast<int64_t>(f.g.ny) * f.g.nx;
@@ -149,32 +120,33 @@ __device__ __forceinline__ void batch_replay_pixel(const BatchField<T>& f,
                                             np, idx0, 0);
   acc = replay_direction<T, FMA, false, BUF>(acc, pos0, x, kernel, kmid, n_bwd, delta, f.codes,
                                              np, idx0, chunks_of(n_fwd));
-  pass_buffer(f, pass)[idx0] = acc;
+  dst[idx0] = acc;
 }
 
 // The 32 x 32 region of a work item as 16 squares of 8 x 8 lanes; (i_base,
 // j_base) is the lane's pixel in the first square.
 template <typename T, bool FMA, bool BRANCHLESS, int POL, int REC, bool BUF>
 __device__ __forceinline__ void batch_trace_region(const BatchField<T>& f,
-                                                   const T* __restrict__ kernel, int32_t i_base,
-                                                   int32_t j_base) {
+                                                   const T* __restrict__ kernel, T* dst,
+                                                   int32_t i_base, int32_t j_base) {
 #pragma unroll 1
   for (int32_t st = 0; st < 16; ++st) {
     const int32_t i0 = i_base + (st >> 2) * 8, j0 = j_base + (st & 3) * 8;
     if (i0 < f.g.ny && j0 < f.g.nx)
-      batch_trace_pixel<T, FMA, BRANCHLESS, POL, REC, BUF>(f, kernel, i0, j0);
+      batch_trace_pixel<T, FMA, BRANCHLESS, POL, REC, BUF>(f, kernel, dst, i0, j0);
   }
 }
 
 template <typename T, bool FMA, bool BUF>
 __device__ __forceinline__ void batch_replay_region(const BatchField<T>& f,
-                                                    const T* __restrict__ kernel, int32_t pass,
-                                                    const uint32_t* delta, int32_t i_base,
-                                                    int32_t j_base) {
+                                                    const T* __restrict__ kernel, const T* src,
+                                                    T* dst, const uint32_t* delta,
+                                                    int32_t i_base, int32_t j_base) {
 #pragma unroll 1
   for (int32_t st = 0; st < 16; ++st) {
     const int32_t i0 = i_base + (st >> 2) * 8, j0 = j_base + (st & 3) * 8;
-    if (i0 < f.g.ny && j0 < f.g.nx) batch_replay_pixel<T, FMA, BUF>(f, kernel, pass, delta, i0, j0);
+    if (i0 < f.g.ny && j0 < f.g.nx)
+      batch_replay_pixel<T, FMA, BUF>(f, kernel, src, dst, delta, i0, j0);
   }
 }
 
@@ -187,12 +159,12 @@ __device__ __forceinline__ int64_t lane0_value(uint64_t x) {
 }
 
 // Persistent kernel: every wave is an independent worker.  A wave pops one
-// work item (a 32 x 32 region of one pass of one field), waits if needed for
-// the field's previous pass, computes the region's pixels as 16 squares of
-// 8 x 8 lanes, then counts the item done.  There is no block barrier in the
-// loop, so waves of one block never wait for each other, and every branch
-// that steers the loop is wave-uniform.
-template <typename T, bool FMA, bool BRANCHLESS, bool BUF>
+// work item (a 32 x 32 region of one pass of one field), waits if needed
+// (DEPS) for the segment it depends on, computes the region's pixels as 16
+// squares of 8 x 8 lanes, then (DEPS) counts the item done.  There is no
+// block barrier in the loop, so waves of one block never wait for each
+// other, and every branch that steers the loop is wave-uniform.
+template <typename T, bool FMA, bool BRANCHLESS, bool BUF, bool DEPS>
 __global__ __launch_bounds__(256) void lic_batch_persistent(const BatchField<T>* __restrict__ fields,
                                                             const Segment* __restrict__ segs,
                                                             const T* __restrict__ kpool,
@@ -265,7 +237,7 @@ __global__ __launch_bounds__(256) void lic_batch_persistent(const BatchField<T>*
     }
     const Segment seg = segs[lo];
     const BatchField<T>& f = fields[seg.field];
-    if (seg.prev >= 0) {
+    if (DEPS && seg.prev >= 0) {
       // every item of this field's previous pass must be complete; those
       // items were queued earlier, hence popped by running waves whose own
       // waits point further back
@@ -295,6 +267,8 @@ __global__ __launch_bounds__(256) void lic_batch_persistent(const BatchField<T>*
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // see the previous pass's stores
         fenced_prev = seg.prev;
       }
+    }
+    if (seg.pass > 0) {
       __builtin_amdgcn_wave_barrier();
       if (lane < 16) delta[lane] = move_delta(lane, f.g, BUF ? sizeof(T) : 1u);
       __builtin_amdgcn_wave_barrier();
@@ -304,70 +278,46 @@ __global__ __launch_bounds__(256) void lic_batch_persistent(const BatchField<T>*
     const int32_t rx = static_cast<int32_t>(local - static_cast<int64_t>(ry) * f.regions_x);
     const int32_t i_base = ry * kRegion + py, j_base = rx * kRegion + px;
     const T* __restrict__ kern = kpool + f.kernel_off;  // uniform: weights via scalar loads
+    T* dst = static_cast<T*>(seg.dst);
     if (seg.pass == 0) {
       // one specialised step loop per (uv_mode, recording) pair, each with
       // its own loop over the region (the branches are uniform)
-      const bool pol = f.uv_mode == 1, rec = f.iterations > 1;
-      if (pol && rec) batch_trace_region<T, FMA, BRANCHLESS, 1, 1, BUF>(f, kern, i_base, j_base);
-      else if (pol) batch_trace_region<T, FMA, BRANCHLESS, 1, 0, BUF>(f, kern, i_base, j_base);
-      else if (rec) batch_trace_region<T, FMA, BRANCHLESS, 0, 1, BUF>(f, kern, i_base, j_base);
-      else batch_trace_region<T, FMA, BRANCHLESS, 0, 0, BUF>(f, kern, i_base, j_base);
+      const bool pol = f.uv_mode == 1, rec = seg.record != 0;
+      if (pol && rec) batch_trace_region<T, FMA, BRANCHLESS, 1, 1, BUF>(f, kern, dst, i_base, j_base);
+      else if (pol) batch_trace_region<T, FMA, BRANCHLESS, 1, 0, BUF>(f, kern, dst, i_base, j_base);
+      else if (rec) batch_trace_region<T, FMA, BRANCHLESS, 0, 1, BUF>(f, kern, dst, i_base, j_base);
+      else batch_trace_region<T, FMA, BRANCHLESS, 0, 0, BUF>(f, kern, dst, i_base, j_base);
     } else {
-      batch_replay_region<T, FMA, BUF>(f, kern, seg.pass, delta, i_base, j_base);
+      batch_replay_region<T, FMA, BUF>(f, kern, static_cast<const T*>(seg.src), dst, delta,
+                                       i_base, j_base);
     }
-    note_done(lo);
+    if (DEPS) note_done(lo);
   }
-  publish();
+  if (DEPS) publish();
 }
 
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
 
-template <typename T>
-static hipError_t launch_persistent(int profile, bool buf, const BatchField<T>* fields,
-                                    const Segment* segs, const T* kpool, int32_t nsegs,
-                                    int64_t total_items,
-                                    unsigned long long* queue, uint32_t* done, uint32_t* failed,
-                                    int64_t wait_ticks, int grid, hipStream_t stream) {
-#define RLIC_BATCH_LAUNCH(FMA, BL, BUF)                                                          \
-  hipLaunchKernelGGL((lic_batch_persistent<T, FMA, BL, BUF>), dim3(grid), dim3(256), 0, stream, \
-                     fields, segs, kpool, nsegs, total_items, queue, done, failed, wait_ticks)
+template <typename T, bool DEPS>
+static const void* persistent_fn(int profile, bool buf) {
   const bool fma = profile & 1, bl = profile & 2;
-  if (buf) {
-    if (fma && bl) RLIC_BATCH_LAUNCH(true, true, true);
-    else if (fma) RLIC_BATCH_LAUNCH(true, false, true);
-    else if (bl) RLIC_BATCH_LAUNCH(false, true, true);
-    else RLIC_BATCH_LAUNCH(false, false, true);
-  } else {
-    if (fma && bl) RLIC_BATCH_LAUNCH(true, true, false);
-    else if (fma) RLIC_BATCH_LAUNCH(true, false, false);
-    else if (bl) RLIC_BATCH_LAUNCH(false, true, false);
-    else RLIC_BATCH_LAUNCH(false, false, false);
-  }
-#undef RLIC_BATCH_LAUNCH
-  return hipGetLastError();
+#define RLIC_PICK(BUF)                                                                           \
+  (fma && bl ? reinterpret_cast<const void*>(&lic_batch_persistent<T, true, true, BUF, DEPS>)    \
+   : fma     ? reinterpret_cast<const void*>(&lic_batch_persistent<T, true, false, BUF, DEPS>)   \
+   : bl      ? reinterpret_cast<const void*>(&lic_batch_persistent<T, false, true, BUF, DEPS>)   \
+             : reinterpret_cast<const void*>(&lic_batch_persistent<T, false, false, BUF, DEPS>))
+  return buf ? RLIC_PICK(true) : RLIC_PICK(false);
+#undef RLIC_PICK
 }
 
-template <typename T>
-static int persistent_grid(int profile, bool buf) {
+// Blocks of a persistent grid that fill the device: CUs x resident blocks per CU.
+static int persistent_grid(const void* fn) {
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 1024;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     cus = 256;
-  const void* fn = nullptr;
-  const bool fma = profile & 1, bl = profile & 2;
-  if (buf) {
-    fn = fma && bl ? reinterpret_cast<const void*>(&lic_batch_persistent<T, true, true, true>)
-         : fma     ? reinterpret_cast<const void*>(&lic_batch_persistent<T, true, false, true>)
-         : bl      ? reinterpret_cast<const void*>(&lic_batch_persistent<T, false, true, true>)
-                   : reinterpret_cast<const void*>(&lic_batch_persistent<T, false, false, true>);
-  } else {
-    fn = fma && bl ? reinterpret_cast<const void*>(&lic_batch_persistent<T, true, true, false>)
-         : fma     ? reinterpret_cast<const void*>(&lic_batch_persistent<T, true, false, false>)
-         : bl      ? reinterpret_cast<const void*>(&lic_batch_persistent<T, false, true, false>)
-                   : reinterpret_cast<const void*>(&lic_batch_persistent<T, false, false, false>);
-  }
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0) != hipSuccess ||
       per_cu < 1) {
     (void)hipGetLastError();
@@ -378,104 +328,56 @@ static int persistent_grid(int profile, bool buf) {
 
 static size_t align_up(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
 
-template <typename T, typename FieldSpec>
-int batch_run(const FieldSpec* specs, int64_t nfields, int profile, hipStream_t stream,
-              BatchScratch& scratch, std::string& err) {
-  // ---- layout of the scratch block
-  std::vector<BatchField<T>> fields(nfields);
-  std::vector<size_t> off_kpad(nfields), off_cells(nfields), off_tmp(nfields), off_codes(nfields);
-  size_t bytes = 0;
-  int64_t pack_blocks = 0;
+template <typename T>
+int launch_queue(std::vector<BatchField<T>>& fields, const std::vector<const T*>& kernel_src,
+                 std::vector<Segment>& segs, bool deps, bool pack, int profile,
+                 hipStream_t stream, std::string& err) {
+  const size_t nfields = fields.size();
+  if (nfields == 0) return RLIC_OK;
+  // per-field regions, pack blocks and weight-pool slots
+  int64_t pack_blocks = 0, pool_len = 0;
   bool buf = true;
-  for (int64_t k = 0; k < nfields; ++k) {
-    const FieldSpec& s = specs[k];
-    const int64_t n = s.ny * s.nx;
-    off_kpad[k] = bytes;
-    bytes = align_up(bytes + sizeof(T) * (s.kernel_len + 2 * kKernelPad));
-    off_cells[k] = bytes;
-    if (s.iterations > 0) bytes = align_up(bytes + sizeof(Cell<T>) * n);
-    off_tmp[k] = bytes;
-    if (s.iterations > 1) bytes = align_up(bytes + sizeof(T) * n);
-    off_codes[k] = bytes;
-    if (s.iterations > 1)
-      bytes = align_up(bytes + sizeof(uint4) * code_chunks(static_cast<int32_t>(s.kernel_len)) * n);
+  for (BatchField<T>& f : fields) {
+    f.regions_x = (f.g.nx + kRegion - 1) / kRegion;
+    f.regions = f.regions_x * ((f.g.ny + kRegion - 1) / kRegion);
+    f.pack_block0 = pack_blocks;
+    if (pack) pack_blocks += static_cast<int64_t>((f.g.nx + 63) / 64) * ((f.g.ny + 3) / 4);
+    f.kernel_off = pool_len + kKernelPad;
+    pool_len += f.klen + 2 * kKernelPad;
+    const int64_t n = static_cast<int64_t>(f.g.ny) * f.g.nx;
     if (n * static_cast<int64_t>(sizeof(Cell<T>)) > 0xFFFFFFFFll) buf = false;
   }
-  // ---- queue segments (pass-major)
-  std::vector<Segment> segs;
-  std::vector<int32_t> last_seg(nfields, -1);
-  int64_t max_it = 0;
-  for (int64_t k = 0; k < nfields; ++k) max_it = std::max<int64_t>(max_it, specs[k].iterations);
-  std::vector<int32_t> regions(nfields), regions_x(nfields);
-  for (int64_t k = 0; k < nfields; ++k) {
-    regions_x[k] = static_cast<int32_t>((specs[k].nx + kRegion - 1) / kRegion);
-    regions[k] = regions_x[k] * static_cast<int32_t>((specs[k].ny + kRegion - 1) / kRegion);
-  }
   int64_t items = 0;
-  for (int64_t pass = 0; pass < max_it; ++pass) {
-    for (int64_t k = 0; k < nfields; ++k) {
-      if (specs[k].iterations <= pass || specs[k].ny * specs[k].nx == 0) continue;
-      Segment sgm;
-      sgm.item0 = items;
-      sgm.field = static_cast<int32_t>(k);
-      sgm.pass = static_cast<int32_t>(pass);
-      sgm.prev = last_seg[k];
-      sgm.pad = 0;
-      last_seg[k] = static_cast<int32_t>(segs.size());
-      segs.push_back(sgm);
-      items += regions[k];
-    }
+  for (Segment& sg : segs) {
+    sg.item0 = items;
+    items += fields[sg.field].regions;
   }
-  const size_t off_fields = bytes;
-  bytes = align_up(bytes + sizeof(BatchField<T>) * nfields);
-  const size_t off_kptrs = bytes;
-  bytes = align_up(bytes + sizeof(const T*) * nfields);
-  const size_t off_segs = bytes;
-  bytes = align_up(bytes + sizeof(Segment) * std::max<size_t>(segs.size(), 1));
-  const size_t off_done = bytes;  // done counters, queue head, failure flag
-  const size_t n_counters = segs.size() + 4;
-  bytes = align_up(bytes + sizeof(uint32_t) * n_counters);
-
-  char* base = nullptr;
+  // one scratch block: fields | kernel pointers | segments | counters | weight pool
+  const size_t off_fields = 0;
+  const size_t off_kptrs = align_up(sizeof(BatchField<T>) * nfields);
+  const size_t off_segs = align_up(off_kptrs + sizeof(const T*) * nfields);
+  const size_t off_done = align_up(off_segs + sizeof(Segment) * std::max<size_t>(segs.size(), 1));
+  const size_t n_counters = segs.size() + 4;  // done per segment, queue head (8 B), failure flag
+  const size_t off_pool = align_up(off_done + sizeof(uint32_t) * n_counters);
+  const size_t bytes = off_pool + sizeof(T) * static_cast<size_t>(pool_len);
+  BatchScratch scratch;
   if (hipError_t e = scratch.reserve(bytes, stream); e != hipSuccess) {
-    err = std::string("allocating batch scratch: ") + hipGetErrorString(e);
+    err = std::string("allocating queue descriptors: ") + hipGetErrorString(e);
     return e == hipErrorOutOfMemory ? RLIC_ERR_OUT_OF_MEMORY : RLIC_ERR_HIP;
   }
-  base = static_cast<char*>(scratch.ptr);
-
-  std::vector<const T*> kptrs(nfields);
-  for (int64_t k = 0; k < nfields; ++k) {
-    const FieldSpec& s = specs[k];
-    BatchField<T>& f = fields[k];
-    f.tex = s.texture; f.tsy = s.tex_sy; f.tsx = s.tex_sx;
-    f.u = s.u; f.usy = s.u_sy; f.usx = s.u_sx;
-    f.v = s.v; f.vsy = s.v_sy; f.vsx = s.v_sx;
-    f.kernel = reinterpret_cast<const T*>(base + off_kpad[k]) + kKernelPad;
-    f.kernel_off = static_cast<int64_t>(off_kpad[k] / sizeof(T)) + kKernelPad;
-    f.out = s.out;
-    f.tmp = s.iterations > 1 ? reinterpret_cast<T*>(base + off_tmp[k]) : nullptr;
-    f.cells = s.iterations > 0 ? reinterpret_cast<Cell<T>*>(base + off_cells[k]) : nullptr;
-    f.codes = s.iterations > 1 ? reinterpret_cast<uint4*>(base + off_codes[k]) : nullptr;
-    f.g = FieldGeom{static_cast<int32_t>(s.ny), static_cast<int32_t>(s.nx), s.x_left, s.x_right,
-                    s.y_left, s.y_right};
-    f.klen = static_cast<int32_t>(s.kernel_len);
-    f.uv_mode = s.uv_mode;
-    f.iterations = static_cast<int32_t>(s.iterations);
-    f.regions_x = regions_x[k];
-    f.regions = regions[k];
-    f.pack_block0 = pack_blocks;
-    pack_blocks += ((s.nx + 63) / 64) * ((s.ny + 3) / 4);
-    kptrs[k] = s.kernel;
-  }
-  // descriptors go up in one copy from a staging block of this host
-  std::vector<char> host(bytes - off_fields);
-  std::memcpy(host.data(), fields.data(), sizeof(BatchField<T>) * nfields);
-  std::memcpy(host.data() + (off_kptrs - off_fields), kptrs.data(), sizeof(const T*) * nfields);
+  char* base = static_cast<char*>(scratch.ptr);
+  T* d_pool = reinterpret_cast<T*>(base + off_pool);
+  for (BatchField<T>& f : fields) f.kernel = d_pool + f.kernel_off;
+  std::vector<char> host(off_done);
+  std::memcpy(host.data() + off_fields, fields.data(), sizeof(BatchField<T>) * nfields);
+  std::memcpy(host.data() + off_kptrs, kernel_src.data(), sizeof(const T*) * nfields);
   if (!segs.empty())
-    std::memcpy(host.data() + (off_segs - off_fields), segs.data(), sizeof(Segment) * segs.size());
-  hipError_t e = hipMemcpyAsync(base + off_fields, host.data(), off_done - off_fields,
-                                hipMemcpyHostToDevice, stream);
-  if (e != hipSuccess) { err = "uploading batch descriptors"; return RLIC_ERR_HIP; }
+    std::memcpy(host.data() + off_segs, segs.data(), sizeof(Segment) * segs.size());
+  hipError_t e = hipMemcpyAsync(base, host.data(), off_done, hipMemcpyHostToDevice, stream);
+  if (e != hipSuccess) {
+    err = std::string("uploading queue descriptors: ") + hipGetErrorString(e);
+    return RLIC_ERR_HIP;
+  }
   auto* d_fields = reinterpret_cast<const BatchField<T>*>(base + off_fields);
   auto* d_kptrs = reinterpret_cast<const T* const*>(base + off_kptrs);
   auto* d_segs = reinterpret_cast<const Segment*>(base + off_segs);
@@ -483,36 +385,41 @@ int batch_run(const FieldSpec* specs, int64_t nfields, int profile, hipStream_t 
   auto* d_queue = reinterpret_cast<unsigned long long*>(d_done + segs.size() + (segs.size() & 1));
   auto* d_failed = reinterpret_cast<uint32_t*>(d_queue + 1);
 
-  hipLaunchKernelGGL(lic_zero_u32, dim3(std::max<int64_t>(1, std::min<int64_t>(
-                                        (n_counters + 255) / 256, 1024))),
+  hipLaunchKernelGGL(lic_zero_u32, dim3(static_cast<unsigned>(std::min<size_t>(
+                                       (n_counters + 255) / 256, 1024))),
                      dim3(256), 0, stream, d_done, static_cast<int64_t>(n_counters));
-  hipLaunchKernelGGL(lic_pad_batch<T>, dim3(nfields), dim3(256), 0, stream, d_fields, d_kptrs);
-  if (pack_blocks > 0) {
-    hipLaunchKernelGGL(lic_pack_batch<T>, dim3(pack_blocks), dim3(256), 0, stream, d_fields,
-                       static_cast<int32_t>(nfields));
+  hipLaunchKernelGGL(lic_pad_batch<T>, dim3(static_cast<unsigned>(nfields)), dim3(256), 0, stream,
+                     d_fields, d_kptrs);
+  if (pack && pack_blocks > 0) {
+    hipLaunchKernelGGL(lic_pack_batch<T>, dim3(static_cast<unsigned>(pack_blocks)), dim3(256), 0,
+                       stream, d_fields, static_cast<int32_t>(nfields));
   }
   if ((e = hipGetLastError()) != hipSuccess) {
-    err = std::string("batch set-up kernels: ") + hipGetErrorString(e);
+    err = std::string("queue set-up kernels: ") + hipGetErrorString(e);
     return RLIC_ERR_HIP;
   }
   if (items > 0) {
-    // four waves (workers) per block; a wait is declared stuck after
-    // RLIC_BATCH_WAIT_MS milliseconds (default 2000) of the device's wall clock
-    const int grid = static_cast<int>(std::min<int64_t>((items + 3) / 4,
-                                                        persistent_grid<T>(profile, buf)));
+    // four waves (workers) per block; with deps, a wait is declared stuck
+    // after RLIC_BATCH_WAIT_MS milliseconds (default 2000) of the device's
+    // wall clock
+    const void* fn = deps ? persistent_fn<T, true>(profile, buf) : persistent_fn<T, false>(profile, buf);
+    const int grid =
+        static_cast<int>(std::min<int64_t>((items + 3) / 4, persistent_grid(fn)));
     int dev = 0, khz = 100000;
     if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) {
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
+        khz <= 0) {
       (void)hipGetLastError();
       khz = 100000;
     }
     const char* env = std::getenv("RLIC_BATCH_WAIT_MS");
-    const int64_t wait_ms = env ? std::max<int64_t>(0, std::atoll(env)) : 2000;
-    e = launch_persistent<T>(profile, buf, d_fields, d_segs, reinterpret_cast<const T*>(base),
-                             static_cast<int32_t>(segs.size()),
-                             items, d_queue, d_done, d_failed, wait_ms * khz, grid, stream);
+    const int64_t wait_ticks = (env ? std::max<int64_t>(0, std::atoll(env)) : 2000) * khz;
+    const int32_t nsegs = static_cast<int32_t>(segs.size());
+    void* args[] = {&d_fields, &d_segs, &d_pool, const_cast<int32_t*>(&nsegs), &items,
+                    &d_queue, &d_done, &d_failed, const_cast<int64_t*>(&wait_ticks)};
+    e = hipLaunchKernel(fn, dim3(grid), dim3(256), args, 0, stream);
     if (e != hipSuccess) {
-      err = std::string("batch kernel: ") + hipGetErrorString(e);
+      err = std::string("queue kernel: ") + hipGetErrorString(e);
       return RLIC_ERR_HIP;
     }
   }
@@ -520,16 +427,91 @@ int batch_run(const FieldSpec* specs, int64_t nfields, int profile, hipStream_t 
   e = hipMemcpyAsync(&failed, d_failed, sizeof(failed), hipMemcpyDeviceToHost, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   if (e != hipSuccess) {
-    err = std::string("batch completion: ") + hipGetErrorString(e);
+    err = std::string("queue completion: ") + hipGetErrorString(e);
     return RLIC_ERR_HIP;
   }
   if (failed) {
-    err = "batch: a work item waited too long for the previous pass of its field";
+    err = "a work item waited too long for the previous pass of its field";
     return RLIC_ERR_HIP;
   }
   return RLIC_OK;
 }
 
+template <typename T, typename FieldSpec>
+int batch_run(const FieldSpec* specs, int64_t nfields, int profile, hipStream_t stream,
+              BatchScratch& scratch, std::string& err) {
+  // per-field working memory: cells, ping-pong partner, move record
+  std::vector<size_t> off_cells(nfields), off_tmp(nfields), off_codes(nfields);
+  size_t bytes = 0;
+  for (int64_t k = 0; k < nfields; ++k) {
+    const FieldSpec& s = specs[k];
+    const size_t n = static_cast<size_t>(s.ny * s.nx);
+    off_cells[k] = bytes;
+    if (s.iterations > 0) bytes = align_up(bytes + sizeof(Cell<T>) * n);
+    off_tmp[k] = bytes;
+    if (s.iterations > 1) bytes = align_up(bytes + sizeof(T) * n);
+    off_codes[k] = bytes;
+    if (s.iterations > 1)
+      bytes = align_up(bytes + sizeof(uint4) * code_chunks(static_cast<int32_t>(s.kernel_len)) * n);
+  }
+  if (hipError_t e = scratch.reserve(bytes, stream); e != hipSuccess) {
+    err = std::string("allocating batch working memory: ") + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? RLIC_ERR_OUT_OF_MEMORY : RLIC_ERR_HIP;
+  }
+  char* base = static_cast<char*>(scratch.ptr);
+  std::vector<BatchField<T>> fields(nfields);
+  std::vector<const T*> ksrc(nfields);
+  std::vector<T*> tmp(nfields);
+  int64_t max_it = 0;
+  for (int64_t k = 0; k < nfields; ++k) {
+    const FieldSpec& s = specs[k];
+    BatchField<T>& f = fields[k];
+    std::memset(&f, 0, sizeof(f));
+    f.tex = s.texture; f.tsy = s.tex_sy; f.tsx = s.tex_sx;
+    f.u = s.u; f.usy = s.u_sy; f.usx = s.u_sx;
+    f.v = s.v; f.vsy = s.v_sy; f.vsx = s.v_sx;
+    f.out = s.out;
+    f.cells = s.iterations > 0 ? reinterpret_cast<Cell<T>*>(base + off_cells[k]) : nullptr;
+    f.codes = s.iterations > 1 ? reinterpret_cast<uint4*>(base + off_codes[k]) : nullptr;
+    f.g = FieldGeom{static_cast<int32_t>(s.ny), static_cast<int32_t>(s.nx), s.x_left, s.x_right,
+                    s.y_left, s.y_right};
+    f.klen = static_cast<int32_t>(s.kernel_len);
+    f.uv_mode = s.uv_mode;
+    f.iterations = static_cast<int32_t>(s.iterations);
+    ksrc[k] = s.kernel;
+    tmp[k] = s.iterations > 1 ? reinterpret_cast<T*>(base + off_tmp[k]) : nullptr;
+    max_it = std::max<int64_t>(max_it, s.iterations);
+  }
+  // pass-major segments; pass p of a field with n passes writes `out` when
+  // n-1-p is even and the ping-pong partner otherwise, so the last pass
+  // lands in `out`
+  std::vector<Segment> segs;
+  std::vector<int32_t> last_seg(nfields, -1);
+  for (int64_t pass = 0; pass < max_it; ++pass) {
+    for (int64_t k = 0; k < nfields; ++k) {
+      const FieldSpec& s = specs[k];
+      if (s.iterations <= pass || s.ny * s.nx == 0) continue;
+      auto buffer = [&](int64_t p) -> T* { return ((s.iterations - 1 - p) % 2 == 0) ? s.out : tmp[k]; };
+      Segment sg{};
+      sg.field = static_cast<int32_t>(k);
+      sg.pass = static_cast<int32_t>(pass);
+      sg.prev = last_seg[k];
+      sg.record = (pass == 0 && s.iterations > 1) ? 1 : 0;
+      sg.src = pass > 0 ? buffer(pass - 1) : nullptr;
+      sg.dst = buffer(pass);
+      last_seg[k] = static_cast<int32_t>(segs.size());
+      segs.push_back(sg);
+    }
+  }
+  return launch_queue<T>(fields, ksrc, segs, /*deps=*/true, /*pack=*/true, profile, stream, err);
+}
+
+template int launch_queue<float>(std::vector<BatchField<float>>&, const std::vector<const float*>&,
+                                 std::vector<Segment>&, bool, bool, int, hipStream_t,
+                                 std::string&);
+template int launch_queue<double>(std::vector<BatchField<double>>&,
+                                  const std::vector<const double*>&, std::vector<Segment>&, bool,
+                                  bool, int, hipStream_t, std::string&);
 template int batch_run<float, rlic_field_f32>(const rlic_field_f32*, int64_t, int, hipStream_t,
                                               BatchScratch&, std::string&);
 template int batch_run<double, rlic_field_f64>(const rlic_field_f64*, int64_t, int, hipStream_t,

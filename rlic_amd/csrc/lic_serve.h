@@ -1,0 +1,91 @@
+// Serving core (lic_serve.hip): fields resident in device memory, advanced
+// one pass per step, many fields per step in one persistent launch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "lic_device.h"
+
+namespace rlic {
+
+// Device state of one admitted field, in one allocation:
+// [weights | cells | buffer 0 | buffer 1 | move record].
+template <typename T>
+struct Resident {
+  FieldGeom g{};
+  int32_t klen = 0, uv_mode = 0;
+  void* block = nullptr;
+  size_t bytes = 0;
+  T* weights = nullptr;
+  Cell<T>* cells = nullptr;
+  T* buf[2] = {nullptr, nullptr};
+  uint4* codes = nullptr;  // nullptr: the field keeps no move record
+  int64_t done = 0;        // passes computed
+  int64_t target = 0;      // passes requested so far
+  int cur = -1;            // buffer holding the latest output (-1: none yet)
+  bool recorded = false;   // `codes` holds the move record
+};
+
+class Server {
+ public:
+  Server(int64_t budget_bytes, int profile, hipStream_t stream)
+      : budget_(budget_bytes), profile_(profile), stream_(stream) {}
+  ~Server();
+  Server(const Server&) = delete;
+  Server& operator=(const Server&) = delete;
+
+  // Bytes one field occupies while resident.
+  template <typename T>
+  static size_t field_bytes(int64_t ny, int64_t nx, int64_t klen, bool record);
+
+  // Copies the kernel weights, packs (u, v, texture) and registers the field
+  // for `iterations` (> 0) passes.  Returns RLIC_ERR_OUT_OF_MEMORY when the
+  // field does not fit the remaining budget (nothing is allocated then).
+  // `record`: keep the move record, for more than one pass or follow-ups.
+  template <typename T>
+  int admit(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t usy, int64_t usx,
+            const T* v, int64_t vsy, int64_t vsx, int64_t ny, int64_t nx, const T* kernel,
+            int64_t klen, int uv_mode, int xl, int xr, int yl, int yr, int64_t iterations,
+            bool record, int64_t* handle, std::string& err);
+
+  // One pass of every field with passes left, in one launch per dtype;
+  // appends the handles whose requested passes are all done.  Synchronous.
+  int step(std::vector<int64_t>& finished, std::string& err);
+
+  // Requests `more` passes of a resident field whose passes are all done
+  // (it must keep its move record, unless it has not run yet).
+  int extend(int64_t handle, int64_t more, std::string& err);
+
+  // Copies the field's latest output (ny x nx, C order) to `out`, device
+  // or host memory.  Synchronous.
+  template <typename T>
+  int read(int64_t handle, T* out, std::string& err);
+
+  int release(int64_t handle, std::string& err);
+
+  void usage(int64_t* used_bytes, int64_t* resident, int64_t* active);
+  hipStream_t stream() const { return stream_; }
+
+ private:
+  template <typename T>
+  std::map<int64_t, Resident<T>>& table();
+  template <typename T>
+  int step_dtype(std::vector<int64_t>& finished, std::string& err);
+
+  std::mutex mu_;
+  int64_t budget_;
+  int64_t used_ = 0;
+  int profile_;
+  hipStream_t stream_;
+  int64_t next_handle_ = 1;
+  std::map<int64_t, Resident<float>> f32_;
+  std::map<int64_t, Resident<double>> f64_;
+};
+
+}  // namespace rlic

@@ -1,0 +1,252 @@
+// Serving core: resident fields advanced one pass per step.
+//
+// A request is one `rlic.convolve` call (src/lib.rs:408-443).  Admission
+// copies its kernel weights and packs (u, v, texture) into device memory
+// that stays resident until the field is released; each step then runs one
+// pass of every field that has passes left, all fields of a dtype in one
+// persistent launch over 32 x 32 regions (launch_queue without waits: the
+// step boundary orders the passes).  A field's first pass traces and, when
+// it keeps a move record, records the moves; later passes (and follow-up
+// passes requested after the field finished) replay them.  The two output
+// buffers alternate, so a field's passes give the same bits as one
+// convolve call with the total number of iterations, whatever else shares
+// the step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "../../include/rlic_hip.h"
+#include "lic_batch.h"
+#include "lic_launch.h"
+#include "lic_serve.h"
+
+namespace rlic {
+
+static size_t align256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
+
+template <>
+std::map<int64_t, Resident<float>>& Server::table<float>() { return f32_; }
+template <>
+std::map<int64_t, Resident<double>>& Server::table<double>() { return f64_; }
+
+Server::~Server() {
+  for (auto& kv : f32_) (void)hipFreeAsync(kv.second.block, stream_);
+  for (auto& kv : f64_) (void)hipFreeAsync(kv.second.block, stream_);
+  (void)hipStreamSynchronize(stream_);
+}
+
+template <typename T>
+size_t Server::field_bytes(int64_t ny, int64_t nx, int64_t klen, bool record) {
+  const size_t n = static_cast<size_t>(ny * nx);
+  size_t b = align256(sizeof(T) * static_cast<size_t>(std::max<int64_t>(klen, 1)));
+  b += align256(sizeof(Cell<T>) * n);
+  b += 2 * align256(sizeof(T) * n);
+  if (record) b += align256(sizeof(uint4) * static_cast<size_t>(code_chunks(static_cast<int32_t>(klen))) * n);
+  return b;
+}
+
+template <typename T>
+int Server::admit(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t usy, int64_t usx,
+                  const T* v, int64_t vsy, int64_t vsx, int64_t ny, int64_t nx, const T* kernel,
+                  int64_t klen, int uv_mode, int xl, int xr, int yl, int yr, int64_t iterations,
+                  bool record, int64_t* handle, std::string& err) {
+  std::lock_guard<std::mutex> lock(mu_);
+  const size_t bytes = field_bytes<T>(ny, nx, klen, record);
+  if (used_ + static_cast<int64_t>(bytes) > budget_) {
+    err = "the field needs " + std::to_string(bytes) + " bytes; " +
+          std::to_string(budget_ - used_) + " of the serving budget are free";
+    return RLIC_ERR_OUT_OF_MEMORY;
+  }
+  Resident<T> r;
+  r.g = FieldGeom{static_cast<int32_t>(ny), static_cast<int32_t>(nx), xl, xr, yl, yr};
+  r.klen = static_cast<int32_t>(klen);
+  r.uv_mode = uv_mode;
+  r.bytes = bytes;
+  if (hipError_t e = hipMallocAsync(&r.block, bytes, stream_); e != hipSuccess) {
+    (void)hipGetLastError();
+    err = std::string("allocating resident field: ") + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? RLIC_ERR_OUT_OF_MEMORY : RLIC_ERR_HIP;
+  }
+  const size_t n = static_cast<size_t>(ny * nx);
+  char* p = static_cast<char*>(r.block);
+  r.weights = reinterpret_cast<T*>(p);
+  p += align256(sizeof(T) * static_cast<size_t>(std::max<int64_t>(klen, 1)));
+  r.cells = reinterpret_cast<Cell<T>*>(p);
+  p += align256(sizeof(Cell<T>) * n);
+  r.buf[0] = reinterpret_cast<T*>(p);
+  p += align256(sizeof(T) * n);
+  r.buf[1] = reinterpret_cast<T*>(p);
+  p += align256(sizeof(T) * n);
+  r.codes = record ? reinterpret_cast<uint4*>(p) : nullptr;
+  r.target = iterations;
+  hipError_t e = hipMemcpyAsync(r.weights, kernel, sizeof(T) * static_cast<size_t>(klen),
+                                hipMemcpyDefault, stream_);
+  if (e == hipSuccess && n > 0)
+    e = launch_pack<T>(tex, tsy, tsx, u, usy, usx, v, vsy, vsx, static_cast<int32_t>(ny),
+                       static_cast<int32_t>(nx), r.cells, stream_);
+  if (e != hipSuccess) {
+    (void)hipFreeAsync(r.block, stream_);
+    err = std::string("admitting field: ") + hipGetErrorString(e);
+    return RLIC_ERR_HIP;
+  }
+  used_ += static_cast<int64_t>(bytes);
+  *handle = next_handle_++;
+  table<T>().emplace(*handle, r);
+  return RLIC_OK;
+}
+
+template <typename T>
+int Server::step_dtype(std::vector<int64_t>& finished, std::string& err) {
+  std::vector<BatchField<T>> fields;
+  std::vector<const T*> ksrc;
+  std::vector<Segment> segs;
+  std::vector<Resident<T>*> stepped;
+  std::vector<int64_t> handles;
+  for (auto& kv : table<T>()) {
+    Resident<T>& r = kv.second;
+    if (r.done >= r.target) continue;
+    if (static_cast<int64_t>(r.g.ny) * r.g.nx == 0) {  // nothing to compute
+      r.done = r.target;
+      finished.push_back(kv.first);
+      continue;
+    }
+    BatchField<T> f;
+    std::memset(&f, 0, sizeof(f));
+    f.cells = r.cells;
+    f.codes = r.codes;
+    f.g = r.g;
+    f.klen = r.klen;
+    f.uv_mode = r.uv_mode;
+    f.iterations = static_cast<int32_t>(std::min<int64_t>(r.target - r.done, INT32_MAX));
+    Segment sg{};
+    sg.field = static_cast<int32_t>(fields.size());
+    sg.prev = -1;
+    const int out_buf = r.cur < 0 ? 0 : 1 - r.cur;
+    sg.dst = r.buf[out_buf];
+    if (r.done == 0) {
+      sg.pass = 0;
+      sg.record = r.codes != nullptr ? 1 : 0;
+      sg.src = nullptr;
+    } else {
+      sg.pass = 1;
+      sg.record = 0;
+      sg.src = r.buf[r.cur];
+    }
+    fields.push_back(f);
+    ksrc.push_back(r.weights);
+    segs.push_back(sg);
+    stepped.push_back(&r);
+    handles.push_back(kv.first);
+  }
+  if (fields.empty()) return RLIC_OK;
+  const int rc = launch_queue<T>(fields, ksrc, segs, /*deps=*/false, /*pack=*/false, profile_,
+                                 stream_, err);
+  if (rc != RLIC_OK) return rc;
+  for (size_t k = 0; k < stepped.size(); ++k) {
+    Resident<T>& r = *stepped[k];
+    if (r.done == 0 && r.codes != nullptr) r.recorded = true;
+    r.cur = r.cur < 0 ? 0 : 1 - r.cur;
+    ++r.done;
+    if (r.done == r.target) finished.push_back(handles[k]);
+  }
+  return RLIC_OK;
+}
+
+int Server::step(std::vector<int64_t>& finished, std::string& err) {
+  std::lock_guard<std::mutex> lock(mu_);
+  int rc = step_dtype<float>(finished, err);
+  if (rc == RLIC_OK) rc = step_dtype<double>(finished, err);
+  return rc;
+}
+
+int Server::extend(int64_t handle, int64_t more, std::string& err) {
+  std::lock_guard<std::mutex> lock(mu_);
+  auto go = [&](auto& tab) -> int {
+    auto it = tab.find(handle);
+    if (it == tab.end()) return 1;
+    auto& r = it->second;
+    if (r.done < r.target) {
+      err = "the field still has passes running";
+      return RLIC_ERR_INVALID_ARGUMENT;
+    }
+    if (r.done > 0 && !r.recorded) {
+      err = "the field keeps no move record: it cannot run more passes";
+      return RLIC_ERR_INVALID_ARGUMENT;
+    }
+    r.target += more;
+    return RLIC_OK;
+  };
+  int rc = go(f32_);
+  if (rc == 1) rc = go(f64_);
+  if (rc == 1) {
+    err = "unknown field handle " + std::to_string(handle);
+    return RLIC_ERR_INVALID_ARGUMENT;
+  }
+  return rc;
+}
+
+template <typename T>
+int Server::read(int64_t handle, T* out, std::string& err) {
+  std::lock_guard<std::mutex> lock(mu_);
+  auto it = table<T>().find(handle);
+  if (it == table<T>().end()) {
+    err = "unknown field handle " + std::to_string(handle) + " for this dtype";
+    return RLIC_ERR_INVALID_ARGUMENT;
+  }
+  const Resident<T>& r = it->second;
+  const size_t n = static_cast<size_t>(r.g.ny) * r.g.nx;
+  if (n == 0) return RLIC_OK;  // an empty field has an empty output
+  if (!out) {
+    err = "null output pointer";
+    return RLIC_ERR_INVALID_ARGUMENT;
+  }
+  if (r.cur < 0) {
+    err = "the field has no output yet";
+    return RLIC_ERR_INVALID_ARGUMENT;
+  }
+  hipError_t e = hipMemcpyAsync(out, r.buf[r.cur], sizeof(T) * n, hipMemcpyDefault, stream_);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+  if (e != hipSuccess) {
+    err = std::string("reading field output: ") + hipGetErrorString(e);
+    return RLIC_ERR_HIP;
+  }
+  return RLIC_OK;
+}
+
+int Server::release(int64_t handle, std::string& err) {
+  std::lock_guard<std::mutex> lock(mu_);
+  auto go = [&](auto& tab) -> bool {
+    auto it = tab.find(handle);
+    if (it == tab.end()) return false;
+    (void)hipFreeAsync(it->second.block, stream_);
+    used_ -= static_cast<int64_t>(it->second.bytes);
+    tab.erase(it);
+    return true;
+  };
+  if (go(f32_) || go(f64_)) return RLIC_OK;
+  err = "unknown field handle " + std::to_string(handle);
+  return RLIC_ERR_INVALID_ARGUMENT;
+}
+
+void Server::usage(int64_t* used_bytes, int64_t* resident, int64_t* active) {
+  std::lock_guard<std::mutex> lock(mu_);
+  int64_t act = 0;
+  for (auto& kv : f32_) act += kv.second.done < kv.second.target;
+  for (auto& kv : f64_) act += kv.second.done < kv.second.target;
+  if (used_bytes) *used_bytes = used_;
+  if (resident) *resident = static_cast<int64_t>(f32_.size() + f64_.size());
+  if (active) *active = act;
+}
+
+#define RLIC_SERVER_INSTANTIATE(T)                                                               \
+  template size_t Server::field_bytes<T>(int64_t, int64_t, int64_t, bool);                       \
+  template int Server::admit<T>(const T*, int64_t, int64_t, const T*, int64_t, int64_t, const T*, \
+                                int64_t, int64_t, int64_t, int64_t, const T*, int64_t, int, int, \
+                                int, int, int, int64_t, bool, int64_t*, std::string&);           \
+  template int Server::read<T>(int64_t, T*, std::string&);
+
+RLIC_SERVER_INSTANTIATE(float)
+RLIC_SERVER_INSTANTIATE(double)
+
+}  // namespace rlic

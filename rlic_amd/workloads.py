@@ -186,6 +186,50 @@ def pixel_steps(fields: list[dict]) -> int:
     return total
 
 
+def serving_pool(n_fields: int = 96, seed: int = 5, device=None, max_side: int = 2048,
+                 min_side: int = 128) -> list[dict]:
+    """Input fields for the serving workload, generated on `device` with
+    torch: sides log-uniform in [min_side, max_side] with aspect ratio
+    within 1:4..4:1, texture U[0,1), u and v sums of 8 Fourier modes;
+    alternately float32 and float64.  Returns dicts with texture, u, v."""
+    import torch
+
+    rng = np.random.default_rng(seed)
+    pool = []
+    for k in range(n_fields):
+        while True:
+            h, w = np.exp(rng.uniform(np.log(min_side), np.log(max_side), size=2)).round()
+            if 0.25 <= h / w <= 4.0:
+                break
+        h, w = int(h), int(w)
+        tdt = torch.float32 if k % 2 == 0 else torch.float64
+        gen = torch.Generator(device=device).manual_seed(int(rng.integers(0, 2**31)))
+        tex = torch.rand((h, w), generator=gen, device=device, dtype=torch.float64).to(tdt)
+        u = _torch_modes(gen, h, w, device).to(tdt)
+        v = _torch_modes(gen, h, w, device).to(tdt)
+        pool.append(dict(texture=tex, u=u, v=v))
+    return pool
+
+
+def serving_request(rng: np.random.Generator, pool: list[dict], max_iterations: int = 20) -> dict:
+    """One request of the serving workload: a pool field with a kernel of
+    odd length uniform over 17..257 (triangular or boxcar), each axis
+    periodic or closed, uv_mode velocity or polarization, iterations
+    uniform in 1..max_iterations."""
+    import torch
+
+    f = pool[int(rng.integers(len(pool)))]
+    klen = int(rng.choice(np.arange(17, 258, 2)))
+    kernel = 1 - np.abs(np.linspace(-1, 1, klen)) if rng.random() < 0.5 else np.ones(klen)
+    bx = "periodic" if rng.random() < 0.5 else "closed"
+    by = "periodic" if rng.random() < 0.5 else "closed"
+    mode = "polarization" if rng.random() < 0.5 else "velocity"
+    iterations = int(rng.integers(1, max_iterations + 1))
+    kern = torch.from_numpy(kernel).to(device=f["texture"].device, dtype=f["texture"].dtype)
+    return dict(texture=f["texture"], u=f["u"], v=f["v"], kernel=kern, uv_mode=mode,
+                boundaries={"x": bx, "y": by}, iterations=iterations)
+
+
 WORKLOADS = {
     "readme": readme_example,
     "plain4096": plain_at_scale,

@@ -1,0 +1,167 @@
+"""Serving: requests batched at pass granularity give, each, the same bits
+as a single convolve call — alone, in a mix, arriving between steps, under
+a tight memory budget, and as follow-ups of kept fields (GPU); the budget
+arithmetic and the C ABI's argument checks (CPU)."""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import pytest
+
+import rlic_amd
+from rlic_amd import _native
+
+
+def _requests(rng, n, max_side=100):
+    reqs = []
+    for k in range(n):
+        dtype = np.float32 if k % 2 else np.float64
+        ny, nx = int(rng.integers(1, max_side)), int(rng.integers(1, max_side))
+        klen = int(rng.integers(1, 60))
+        tex = rng.random((ny, nx)).astype(dtype)
+        u = (rng.random((ny, nx)) - 0.5).astype(dtype)
+        v = (rng.random((ny, nx)) - 0.5).astype(dtype)
+        if k % 5 == 3:
+            u[rng.random((ny, nx)) < 0.05] = np.nan
+        reqs.append(dict(
+            texture=tex, u=u, v=v, kernel=rng.random(klen).astype(dtype),
+            uv_mode=("velocity", "polarization")[k % 3 == 0],
+            boundaries={"x": ("periodic", "closed")[k % 2], "y": ("closed", "periodic")[k % 4 < 2]},
+            iterations=int(rng.integers(0, 6)),
+        ))
+    return reqs
+
+
+def _expected(r, iterations=None):
+    return rlic_amd.convolve(r["texture"], r["u"], r["v"], kernel=r["kernel"],
+                             uv_mode=r["uv_mode"], boundaries=r["boundaries"],
+                             iterations=r["iterations"] if iterations is None else iterations)
+
+
+def _submit(server, r, **kw):
+    return server.submit(r["texture"], r["u"], r["v"], kernel=r["kernel"], uv_mode=r["uv_mode"],
+                         boundaries=r["boundaries"], iterations=r["iterations"], **kw)
+
+
+def test_field_bytes_arithmetic():
+    lib = _native.load()
+    # 64 x 64 f32, L = 33: weights 256 + cells 64 KiB + two buffers 16 KiB each
+    assert lib.rlic_server_field_bytes_f32(64, 64, 33, 0, 1) == 256 + 65536 + 2 * 16384
+    # keeping the move record adds 2 chunks (16 + 16 steps) of 16 B per pixel
+    assert lib.rlic_server_field_bytes_f32(64, 64, 33, 1, 1) == 256 + 65536 + 2 * 16384 + 131072
+    assert lib.rlic_server_field_bytes_f32(64, 64, 33, 0, 2) == 256 + 65536 + 2 * 16384 + 131072
+    assert lib.rlic_server_field_bytes_f64(10, 10, 1, 0, 1) == 256 + 3328 + 2 * 1024
+    assert lib.rlic_server_field_bytes_f32(-1, 4, 3, 0, 1) == -1
+
+
+def test_server_abi_argument_errors():
+    lib = _native.load()
+    assert not lib.rlic_server_create(-1, 3, None)
+    assert "negative budget" in _native.last_error()
+    assert not lib.rlic_server_create(1 << 20, 9, None)
+    assert "unknown arithmetic profile" in _native.last_error()
+    n = ctypes.c_int64(0)
+    assert lib.rlic_server_step(None, None, 0, ctypes.byref(n)) < 0
+    assert lib.rlic_server_release(None, 1) < 0
+    assert lib.rlic_server_extend(None, 1, 1) < 0
+
+
+@pytest.mark.gpu
+def test_requests_alone_and_in_a_mix_give_the_same_bits(gpu_device):
+    from rlic_amd.serve import Server
+
+    rng = np.random.default_rng(11)
+    reqs = _requests(rng, 14)
+    with Server(1 << 30, device=gpu_device) as mixed:
+        handles = [_submit(mixed, r) for r in reqs[:8]]
+        mixed.step()  # the rest arrive while the first ones run
+        handles += [_submit(mixed, r) for r in reqs[8:]]
+        mixed.run()
+    for k, (r, h) in enumerate(zip(reqs, handles)):
+        assert h.state == "done", k
+        with Server(1 << 30, device=gpu_device) as alone:
+            a = _submit(alone, r)
+            alone.run()
+        exp = _expected(r)
+        assert np.array_equal(a.result, exp, equal_nan=True), f"request {k} alone"
+        assert np.array_equal(h.result, exp, equal_nan=True), f"request {k} in the mix"
+        assert h.admitted is not None and h.finished >= h.admitted >= h.arrival
+
+
+@pytest.mark.gpu
+def test_follow_ups_continue_kept_fields(gpu_device):
+    from rlic_amd.serve import Server
+
+    rng = np.random.default_rng(12)
+    reqs = [r for r in _requests(rng, 6) if r["iterations"] > 0]
+    with Server(1 << 30, device=gpu_device) as srv:
+        first = [_submit(srv, r, keep=True) for r in reqs]
+        srv.run()
+        more = [srv.follow_up(h, 3) for h in first]
+        srv.run()
+        again = [srv.follow_up(h, 1) for h in more]
+        srv.run()
+        assert srv.readmissions == 0
+    for k, r in enumerate(reqs):
+        n = r["iterations"]
+        assert np.array_equal(first[k].result, _expected(r, n), equal_nan=True), k
+        assert np.array_equal(more[k].result, _expected(r, n + 3), equal_nan=True), k
+        assert np.array_equal(again[k].result, _expected(r, n + 4), equal_nan=True), k
+
+
+@pytest.mark.gpu
+def test_tight_budget_admits_in_turn_and_evicts(gpu_device):
+    from rlic_amd.serve import Server
+
+    rng = np.random.default_rng(13)
+    reqs = [r for r in _requests(rng, 10, max_side=64) if r["iterations"] > 0]
+    lib = _native.load()
+    biggest = max(lib.rlic_server_field_bytes_f64(64, 64, 60, 1, 2) for _ in reqs)
+    budget = 2 * biggest  # room for about two fields at a time
+    with Server(budget, device=gpu_device) as srv:
+        hs = [_submit(srv, r, keep=True) for r in reqs]
+        peak = 0
+        while srv.queued or srv.active:
+            srv.step()
+            peak = max(peak, srv.usage()["used_bytes"])
+        assert peak <= budget
+        assert srv.evictions > 0
+        # following up a request whose field was evicted: it is admitted
+        # again from its inputs
+        k_ev = next(k for k, h in enumerate(hs) if not srv.is_resident(h))
+        f = srv.follow_up(hs[k_ev], 2)
+        srv.run()
+        assert srv.readmissions == 1
+    for k, (r, h) in enumerate(zip(reqs, hs)):
+        assert np.array_equal(h.result, _expected(r), equal_nan=True), k
+    exp = _expected(reqs[k_ev], reqs[k_ev]["iterations"] + 2)
+    assert np.array_equal(f.result, exp, equal_nan=True)
+
+
+@pytest.mark.gpu
+def test_device_tensors_and_edge_cases(gpu_device):
+    import torch
+
+    from rlic_amd.serve import Server
+
+    rng = np.random.default_rng(14)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(gpu_device)  # noqa: E731
+    tex = rng.random((40, 30))
+    u = rng.random((40, 30)) - 0.5
+    v = rng.random((40, 30)) - 0.5
+    k = rng.random(9)
+    with Server(1 << 28, device=gpu_device) as srv:
+        a = srv.submit(t(tex), t(u), t(v), kernel=t(k), iterations=3)
+        b = srv.submit(t(tex), t(u), t(v), kernel=t(k), iterations=0)  # a copy, no work
+        c = srv.submit(t(np.ones((0, 5))), t(np.ones((0, 5))), t(np.ones((0, 5))),
+                       kernel=t(k), iterations=2)  # empty field
+        with pytest.raises(ValueError, match="^Found invalid texture element"):
+            srv.submit(t(-tex), t(u), t(v), kernel=t(k))
+        srv.run()
+    assert isinstance(a.result, torch.Tensor) and a.result.device == torch.device(gpu_device)
+    exp = rlic_amd.convolve(tex, u, v, kernel=k, iterations=3)
+    assert np.array_equal(a.result.cpu().numpy(), exp)
+    assert np.array_equal(b.result.cpu().numpy(), tex)
+    assert c.result.shape == (0, 5)
