@@ -77,6 +77,20 @@ def _any_non_finite(arr: Any) -> bool:
     return bool(np.any(~np.isfinite(arr)))
 
 
+def _value_flags(texture: Any, kernel: Any) -> tuple[bool, bool]:
+    """(texture has a negative element, kernel has a non-finite weight);
+    device tensors are reduced on the device and read back together."""
+    if _is_torch(texture) and _is_torch(kernel) and texture.device == kernel.device \
+            and texture.device.type != "cpu" and not texture.is_complex() \
+            and not kernel.is_complex():
+        import torch
+
+        flags = torch.stack([(texture < 0).any(), (~torch.isfinite(kernel)).any()])
+        neg, nonfinite = flags.tolist()
+        return bool(neg), bool(nonfinite)
+    return _any_negative(texture), _any_non_finite(kernel)
+
+
 def _copy(arr: Any) -> Any:
     return arr.clone() if _is_torch(arr) else arr.copy()
 
@@ -159,6 +173,9 @@ def validate(texture, u, v, *, kernel, uv_mode, boundaries, iterations, check_va
     run those checks themselves.
     """
     problems: list[Exception] = []
+    neg_texture = nonfinite_kernel = False
+    if check_values:
+        neg_texture, nonfinite_kernel = _value_flags(texture, kernel)
     if iterations < 0:
         problems.append(
             ValueError(
@@ -190,7 +207,7 @@ def validate(texture, u, v, *, kernel, uv_mode, boundaries, iterations, check_va
         problems.append(
             ValueError(f"Expected a texture with exactly two dimensions. Got texture.ndim={texture.ndim}")
         )
-    if check_values and _any_negative(texture):
+    if neg_texture:
         problems.append(
             ValueError("Found invalid texture element(s). Expected only positive values.")
         )
@@ -206,7 +223,7 @@ def validate(texture, u, v, *, kernel, uv_mode, boundaries, iterations, check_va
         problems.append(
             ValueError(f"Expected a kernel with exactly one dimension. Got kernel.ndim={kernel.ndim}")
         )
-    if check_values and _any_non_finite(kernel):
+    if nonfinite_kernel:
         problems.append(ValueError("Found non-finite value(s) in kernel."))
 
     bs = BoundarySet.from_spec(boundaries)
