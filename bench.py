@@ -348,6 +348,16 @@ def main() -> int:
                                    for k, v in prof.items()},
         }
 
+    # divergence (untimed): lane work of one tracing pass, read back from its
+    # move record (RLIC_TRACE_STATS=1)
+    os.environ["RLIC_TRACE_STATS"] = "1"
+    _native.trace_stats_read()
+    step()
+    torch.cuda.synchronize()
+    divergence = _native.trace_stats_read()
+    del os.environ["RLIC_TRACE_STATS"]
+    log(f"bench: trace lane efficiency {divergence['lane_efficiency']}")
+
     cpu = None
     if args.cpu_rows > 0:
         log(f"bench: CPU oracle baseline on {args.cpu_rows} rows")
@@ -377,6 +387,7 @@ def main() -> int:
             "pixel_steps_per_step": wl.pixel_steps,
         },
         "roofline": roofline,
+        "divergence": divergence,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -195,6 +195,16 @@ const char* rlic_last_error(void);
 int rlic_profiler_enable(int on);
 int64_t rlic_profiler_read(char* buf, int64_t len);
 
+/* Divergence measurement.  No counterpart in the reference (a serial loop,
+ * src/lib.rs:373-374).  With the environment variable RLIC_TRACE_STATS=1,
+ * every tracing pass of rlic_convolve_* also records its moves and reads
+ * them back to count, over the lanes of its waves, the steps each lane's
+ * streamline was alive and the steps its wave executed (a wave runs until
+ * all its lanes' lines have ended).  rlic_trace_stats_read returns the two
+ * sums accumulated since the last read and resets them; their ratio is the
+ * lane efficiency lost to uneven streamline termination. */
+int rlic_trace_stats_read(int64_t* live_lane_steps, int64_t* executed_lane_steps);
+
 /* Library version string.  No counterpart in the reference. */
 const char* rlic_version(void);
 

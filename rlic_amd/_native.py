@@ -91,6 +91,7 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "rlic_convolve_f32": (_CONVOLVE_ARGTYPES, _int),
     "rlic_convolve_f64": (_CONVOLVE_ARGTYPES, _int),
     "rlic_last_error": ([], ctypes.c_char_p),
+    "rlic_trace_stats_read": ([_i64p, _i64p], _int),
     "rlic_profiler_enable": ([_int], _int),
     "rlic_profiler_read": ([ctypes.c_char_p, _i64], _i64),
     "rlic_version": ([], ctypes.c_char_p),
@@ -182,6 +183,16 @@ def profiler_read() -> dict:
     buf = ctypes.create_string_buffer(int(need) + 256)
     lib.rlic_profiler_read(buf, len(buf))
     return json.loads(buf.value.decode())
+
+
+def trace_stats_read() -> dict:
+    """Live and executed lane-steps of the tracing passes run with
+    RLIC_TRACE_STATS=1 since the last read (then reset)."""
+    live, executed = ctypes.c_int64(), ctypes.c_int64()
+    load().rlic_trace_stats_read(ctypes.byref(live), ctypes.byref(executed))
+    eff = live.value / executed.value if executed.value else None
+    return {"live_lane_steps": live.value, "executed_lane_steps": executed.value,
+            "lane_efficiency": eff}
 
 
 def raise_for_status(code: int) -> None:

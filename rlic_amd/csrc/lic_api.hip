@@ -69,6 +69,11 @@ bool g_prof_on = false;
 std::vector<TimedLaunch> g_prof_pending;
 std::map<std::string, std::pair<int64_t, double>> g_prof_done;  // name -> (count, total ms)
 
+// Lane work of tracing passes (RLIC_TRACE_STATS=1): live and executed
+// lane-steps summed over the calls since the last read.
+std::mutex g_stats_mu;
+int64_t g_stats_useful = 0, g_stats_executed = 0;
+
 struct LaunchTimer {
   const char* name;
   hipStream_t stream;
@@ -294,7 +299,8 @@ int run_on_device(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
   }
   const rlic::TileMap m = rlic::make_tile_map(ny, nx, env_int("RLIC_TILE_BAND", 16),
                                               env_int("RLIC_WAVE_W_LOG2", 3));
-  if (iterations == 1) {
+  const bool stats = env_int("RLIC_TRACE_STATS", 0) != 0;
+  if (iterations == 1 && !stats) {
     LaunchTimer timer("lic_trace", stream);
     RLIC_HIP_TRY(rlic::launch_trace<T>(c, kernel, klen, g, m, uv_mode, profile, out, nullptr,
                                        stream),
@@ -335,6 +341,22 @@ int run_on_device(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
           rlic::launch_trace<T>(c, kernel, klen, g, m, uv_mode, profile, a, codes, stream),
           "trace");
     }
+    if (stats) {  // measurement: lane work of the pass, from its move record
+      DevMem st;
+      unsigned long long host_st[2] = {0, 0};
+      RLIC_HIP_TRY(st.alloc(sizeof(host_st), stream), "allocating trace statistics");
+      RLIC_HIP_TRY(hipMemsetAsync(st.ptr, 0, sizeof(host_st), stream), "clearing statistics");
+      RLIC_HIP_TRY(rlic::launch_record_stats(codes, klen, g, m,
+                                             static_cast<unsigned long long*>(st.ptr), stream),
+                   "trace statistics");
+      RLIC_HIP_TRY(hipMemcpyAsync(host_st, st.ptr, sizeof(host_st), hipMemcpyDeviceToHost, stream),
+                   "reading statistics");
+      RLIC_HIP_TRY(hipStreamSynchronize(stream), "reading statistics");
+      std::lock_guard<std::mutex> lock(g_stats_mu);
+      g_stats_useful += static_cast<int64_t>(host_st[0]);
+      g_stats_executed += static_cast<int64_t>(host_st[1]);
+    }
+    if (iterations == 1) return RLIC_OK;  // recorded only for the statistics
     // passes 2..n: direct launches, or one captured graph with RLIC_GRAPH=1
     // (measured no faster on MI355X: kept as an option)
     if (env_int("RLIC_GRAPH", 0) != 0 &&
@@ -738,6 +760,14 @@ int64_t rlic_profiler_read(char* buf, int64_t len) {
     g_prof_done.clear();  // handed over: start a new record
   }
   return need;
+}
+
+int rlic_trace_stats_read(int64_t* live_lane_steps, int64_t* executed_lane_steps) {
+  std::lock_guard<std::mutex> lock(g_stats_mu);
+  if (live_lane_steps) *live_lane_steps = g_stats_useful;
+  if (executed_lane_steps) *executed_lane_steps = g_stats_executed;
+  g_stats_useful = g_stats_executed = 0;
+  return RLIC_OK;
 }
 
 const char* rlic_version(void) { return "0.1.0"; }

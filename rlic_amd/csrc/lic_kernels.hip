@@ -91,6 +91,61 @@ __global__ __launch_bounds__(256) void lic_trace(const Cell<T>* __restrict__ cel
   out[static_cast<int64_t>(i0) * g.nx + j0] = acc;
 }
 
+// Lane work of a tracing pass, read back from its move record.  The trace
+// kernel runs a wave in groups of 8 steps until every lane's line has ended
+// (or the steps run out); a lane whose line has ended keeps executing the
+// group with its position frozen.  Per wave and direction this kernel finds
+// each lane's first stop code (the step where its line ended), and adds
+// the lanes' live steps to stats[0] and the steps the wave executed, times
+// the lanes that hold a pixel, to stats[1].  Same block and lane mapping as
+// lic_trace.
+__global__ __launch_bounds__(256) void lic_record_stats(const uint4* __restrict__ codes,
+                                                        int32_t klen, FieldGeom g, TileMap m,
+                                                        unsigned long long* __restrict__ stats) {
+  int32_t ty, tx, py, px;
+  tile_of_block(blockIdx.x, m, ty, tx);
+  pixel_of_thread(threadIdx.x, m, py, px);
+  const int32_t i0 = ty * tile_h(m) + py, j0 = tx * tile_w(m) + px;
+  const bool has_pixel = i0 < g.ny && j0 < g.nx;
+  const int64_t np = static_cast<int64_t>(g.ny) * g.nx;
+  const int64_t p = has_pixel ? static_cast<int64_t>(i0) * g.nx + j0 : 0;
+  const int32_t kmid = klen / 2;
+  const int32_t lanes = __popcll(__builtin_amdgcn_ballot_w64(has_pixel));
+  unsigned long long useful = 0, executed = 0;
+  for (int dir = 0; dir < 2; ++dir) {
+    const int32_t nsteps = dir == 0 ? klen - 1 - kmid : kmid;
+    const int32_t c0 = dir == 0 ? 0 : chunks_of(klen - 1 - kmid);
+    int32_t end = nsteps;  // first step whose code is the stop code
+    if (has_pixel) {
+      for (int32_t ch = 0; ch < chunks_of(nsteps) && end == nsteps; ++ch) {
+        const uint4 c = codes[(c0 + ch) * np + p];
+        const uint32_t words[4] = {c.x, c.y, c.z, c.w};
+        for (int wi = 0; wi < 4 && end == nsteps; ++wi) {
+          for (int q = 0; q < 8; ++q) {
+            const int32_t s = ch * kChunkSteps + 8 * wi + q;
+            if (s < nsteps && ((words[wi] >> (4 * q)) & 15u) == kMoveStop) {
+              end = s;
+              break;
+            }
+          }
+        }
+      }
+    }
+    if (has_pixel) useful += static_cast<unsigned long long>(end);
+    // the wave's last group is the one where its last line ended
+    int32_t last_group = (has_pixel && end == nsteps) ? INT32_MAX : (has_pixel ? end / 8 : -1);
+    for (int off = 32; off > 0; off >>= 1) last_group = max(last_group, __shfl_xor(last_group, off));
+    const int64_t wave_steps =
+        last_group < 0 ? 0 : min<int64_t>(nsteps, 8 * (static_cast<int64_t>(last_group) + 1));
+    executed += static_cast<unsigned long long>(wave_steps) * lanes;
+  }
+  for (int off = 32; off > 0; off >>= 1) useful += __shfl_xor(useful, off);
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&stats[0], useful);
+    atomicAdd(&stats[1], executed);
+  }
+}
+
 template <typename T, bool FMA, bool BUF>
 __global__ __launch_bounds__(256) void lic_replay(const T* __restrict__ xin,
                                                   const T* __restrict__ kernel, int32_t klen,
@@ -442,6 +497,13 @@ hipError_t launch_replay(const T* x, const T* kernel, int32_t klen, const FieldG
                          stream, x, kernel, klen, g, m, out, codes);
     }
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_record_stats(const uint4* codes, int32_t klen, const FieldGeom& g,
+                               const TileMap& m, unsigned long long* stats, hipStream_t stream) {
+  hipLaunchKernelGGL(lic_record_stats, dim3(m.n_tiles), dim3(kBlock), 0, stream, codes, klen, g,
+                     m, stats);
   return hipGetLastError();
 }
 

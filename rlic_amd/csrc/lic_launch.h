@@ -48,6 +48,11 @@ hipError_t launch_replay(const T* x, const T* kernel, int32_t klen, const FieldG
                          const TileMap& m, int profile, T* out, const uint4* codes,
                          hipStream_t stream);
 
+// Adds the live lane-steps (stats[0]) and executed lane-steps (stats[1]) of
+// a tracing pass with tile map m, read from its move record.
+hipError_t launch_record_stats(const uint4* codes, int32_t klen, const FieldGeom& g,
+                               const TileMap& m, unsigned long long* stats, hipStream_t stream);
+
 // LDS-staged replay: tiles of 64x64, 32x64 or 32x32 pixels, the largest
 // whose box (tile plus kmid halo) fits the device's per-block LDS.
 struct LdsTile {

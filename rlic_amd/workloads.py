@@ -101,6 +101,30 @@ def plain_at_scale(n: int = 4096, dtype=np.float32, iterations: int = 5,
                     kernel, {"x": "periodic", "y": "closed"}, "velocity", iterations)
 
 
+def masked_at_scale(n: int = 4096, dtype=np.float32, iterations: int = 5, seed: int = 6,
+                    n_holes: int = 24) -> Workload:
+    """The plain 4096^2 workload with NaN holes: u and v are NaN inside
+    `n_holes` random disks (radius 40..240 px, about 10% of the pixels), so
+    streamlines end at the holes' edges (src/lib.rs:336-338) after uneven
+    numbers of steps: the case that makes lanes of a wave finish unevenly."""
+    wl = plain_at_scale(n, dtype, iterations)
+    rng = np.random.default_rng(seed)
+    ii = np.arange(n)[:, None]
+    jj = np.arange(n)[None, :]
+    mask = np.zeros((n, n), dtype=bool)
+    for _ in range(n_holes):
+        ci, cj = rng.integers(0, n, size=2)
+        r = rng.uniform(40, 240)
+        mask |= (ii - ci) ** 2 + (jj - cj) ** 2 < r * r
+    u = wl.u.copy()
+    v = wl.v.copy()
+    u[mask] = np.nan
+    v[mask] = np.nan
+    return Workload(f"masked_{n}x{n}_{np.dtype(dtype).name}_L129_it{iterations}", wl.texture, u, v,
+                    wl.kernel, wl.boundaries, wl.uv_mode, iterations,
+                    meta={"nan_fraction": float(mask.mean())})
+
+
 def polarization_field(n: int = 16384, dtype=np.float32, seed: int = 3) -> Workload:
     rng = np.random.default_rng(seed)
     texture = rng.random((n, n), dtype=np.float64).astype(dtype)
@@ -233,6 +257,7 @@ def serving_request(rng: np.random.Generator, pool: list[dict], max_iterations: 
 WORKLOADS = {
     "readme": readme_example,
     "plain4096": plain_at_scale,
+    "masked4096": masked_at_scale,
     "pol16384": lambda: polarization_field(16384, np.float32),
     "pol8192f64": lambda: polarization_field(8192, np.float64),
 }
