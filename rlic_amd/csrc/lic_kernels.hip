@@ -275,11 +275,12 @@ __device__ __forceinline__ void replay_directions_lds(
           pos[e] += static_cast<uint32_t>(static_cast<int32_t>(d2) >> 16);
           xv[2 * h + 1] = lds_at<T>(box, pos[e]);
         }
+        // branch-free over the group (see replay_direction)
+        const int32_t use = min(valid, lim);
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-          if (q >= lim) break;  // uniform: last group only
           const T next_acc = macc<FMA>(kw[q], xv[q], acc[e]);
-          acc[e] = q < valid ? next_acc : acc[e];
+          acc[e] = q < use ? next_acc : acc[e];
         }
         alive[e] = valid == 8;
       }

@@ -176,12 +176,14 @@ __device__ __forceinline__ T replay_direction(T acc, uint32_t pos,
       T xv[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) xv[q] = x(at[q]);
-      const int32_t lim = nsteps - s0;
+      // no break for the last group: a branch here lets the compiler sink
+      // the gathers into it and wait for each one in turn; steps past the
+      // end read "stay" codes (position unchanged, weight from the padding)
+      const int32_t lim = min(valid, nsteps - s0);
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        if (q >= lim) break;  // uniform: last group only
         const T next_acc = macc<FMA>(kw[q], xv[q], acc);
-        acc = q < valid ? next_acc : acc;
+        acc = q < lim ? next_acc : acc;
       }
       alive = valid == 8;
     }
