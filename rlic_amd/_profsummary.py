@@ -4,6 +4,7 @@
     python -m rlic_amd._profsummary traffic <FETCH_SIZE counter_collection.csv>
                                             <WRITE_SIZE counter_collection.csv>
                                             [--workload NAME] [--out-dir DIR]
+    python -m rlic_amd._profsummary counters <counter_collection.csv> ...
 
 `stats` prints the per-kernel launch count and average duration.
 `traffic` turns the per-dispatch FETCH_SIZE / WRITE_SIZE counters (KiB)
@@ -12,6 +13,12 @@ library's copy kernel (lic_copy2d, run by ``bench.py --calibrate-copy`` on
 2^28 float32 = 1 GiB read and 1 GiB written per dispatch): factor = known
 bytes / counter bytes.  With --out-dir, writes traffic_<kernel>_<workload>.json
 files that bench.py reports as roofline.traffic.
+`counters` prints, per kernel, the ratios of the counters present in the
+given runs: VALU lane utilization SQ_THREAD_CYCLES_VALU /
+(64 * SQ_ACTIVE_INST_VALU), LDS bank-conflict share SQ_LDS_BANK_CONFLICT /
+SQ_LDS_IDX_ACTIVE, L2 hit rate TCC_HIT / (TCC_HIT + TCC_MISS), L1 hit rate
+1 - TCP_TCC_READ_REQ / TCP_TOTAL_ACCESSES, and VALU / LDS instructions per
+wave.
 """
 
 from __future__ import annotations
@@ -89,6 +96,36 @@ def traffic(fetch_csv: Path, write_csv: Path) -> dict[str, dict]:
     return result
 
 
+def counter_ratios(paths: list[Path]) -> dict[str, dict]:
+    """Per kernel: counter totals over all dispatches and derived ratios."""
+    totals: dict[str, dict[str, float]] = defaultdict(lambda: defaultdict(float))
+    dispatches: dict[str, set] = defaultdict(set)
+    for path in paths:
+        with open(path, newline="") as f:
+            for row in csv.DictReader(f):
+                name = short_name(row["Kernel_Name"])
+                counter = row["Counter_Name"].removesuffix("_sum")
+                totals[name][counter] += float(row["Counter_Value"])
+                dispatches[name].add((str(path), row["Dispatch_Id"]))
+    out = {}
+    for name, c in totals.items():
+        r: dict[str, float] = {}
+        if c.get("SQ_ACTIVE_INST_VALU"):
+            r["valu_lane_utilization"] = c["SQ_THREAD_CYCLES_VALU"] / (64 * c["SQ_ACTIVE_INST_VALU"])
+        if c.get("SQ_LDS_IDX_ACTIVE"):
+            r["lds_bank_conflict_share"] = c["SQ_LDS_BANK_CONFLICT"] / c["SQ_LDS_IDX_ACTIVE"]
+        if c.get("TCC_HIT", 0) + c.get("TCC_MISS", 0):
+            r["l2_hit_rate"] = c["TCC_HIT"] / (c["TCC_HIT"] + c["TCC_MISS"])
+        if c.get("TCP_TOTAL_ACCESSES"):
+            r["l1_hit_rate"] = 1 - c["TCP_TCC_READ_REQ"] / c["TCP_TOTAL_ACCESSES"]
+        if c.get("SQ_WAVES"):
+            for k in ("SQ_INSTS_VALU", "SQ_INSTS_LDS"):
+                if k in c:
+                    r[k.lower().replace("sq_insts_", "") + "_per_wave"] = c[k] / c["SQ_WAVES"]
+        out[name] = {"ratios": r, "totals": dict(c), "dispatches": len(dispatches[name])}
+    return out
+
+
 def main(argv: list[str] | None = None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -99,7 +136,16 @@ def main(argv: list[str] | None = None) -> int:
     t.add_argument("write_csv", type=Path)
     t.add_argument("--workload", default="plain4096")
     t.add_argument("--out-dir", type=Path)
+    c = sub.add_parser("counters")
+    c.add_argument("csv", type=Path, nargs="+")
     args = ap.parse_args(argv)
+    if args.cmd == "counters":
+        for name, r in sorted(counter_ratios(args.csv).items()):
+            if not name.startswith("lic_"):
+                continue
+            ratios = ", ".join(f"{k} {v:.4g}" for k, v in r["ratios"].items())
+            print(f"{name:24s} {ratios}")
+        return 0
     if args.cmd == "stats":
         for name, agg in sorted(read_stats(args.csv).items(), key=lambda kv: -kv[1]["total_ns"]):
             print(f"{name:24s} calls {agg['calls']:6d}  avg {agg['avg_ms']:10.4f} ms  "
