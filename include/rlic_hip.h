@@ -205,6 +205,23 @@ int64_t rlic_profiler_read(char* buf, int64_t len);
  * lane efficiency lost to uneven streamline termination. */
 int rlic_trace_stats_read(int64_t* live_lane_steps, int64_t* executed_lane_steps);
 
+/* Replay autotuner.  No counterpart in the reference (one CPU loop).  The
+ * first rlic_convolve_* call with at least 3 passes on at least 2^18 pixels
+ * for a new (dtype, shape, kernel length, profile) times its second pass
+ * under each candidate plan — LDS-staged replay with every tile whose box
+ * (tile + kernel-half-length halo) fits the LDS, and L2-gather replay with
+ * two tile strip heights — and later calls with that key use the fastest.
+ * Every plan gives the same bits.  RLIC_AUTOTUNE=0 disables tuning;
+ * RLIC_AUTOTUNE_CACHE=<file> loads the table from and appends it to a text
+ * file.  rlic_autotune_read writes the table as a JSON list (same buffer
+ * convention as rlic_profiler_read); rlic_autotune_load adds the plan lines
+ * of such a file (returns how many, or a negative status); a loaded plan
+ * that cannot run on this device is ignored; rlic_autotune_clear empties
+ * the table. */
+int64_t rlic_autotune_read(char* buf, int64_t len);
+int64_t rlic_autotune_load(const char* path);
+int rlic_autotune_clear(void);
+
 /* Library version string.  No counterpart in the reference. */
 const char* rlic_version(void);
 

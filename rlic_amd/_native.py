@@ -92,6 +92,9 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "rlic_convolve_f64": (_CONVOLVE_ARGTYPES, _int),
     "rlic_last_error": ([], ctypes.c_char_p),
     "rlic_trace_stats_read": ([_i64p, _i64p], _int),
+    "rlic_autotune_read": ([ctypes.c_char_p, _i64], _i64),
+    "rlic_autotune_clear": ([], _int),
+    "rlic_autotune_load": ([ctypes.c_char_p], _i64),
     "rlic_profiler_enable": ([_int], _int),
     "rlic_profiler_read": ([ctypes.c_char_p, _i64], _i64),
     "rlic_version": ([], ctypes.c_char_p),
@@ -183,6 +186,29 @@ def profiler_read() -> dict:
     buf = ctypes.create_string_buffer(int(need) + 256)
     lib.rlic_profiler_read(buf, len(buf))
     return json.loads(buf.value.decode())
+
+
+def autotune_table() -> list:
+    """The replay autotuner's table (one entry per tuned key)."""
+    import json
+
+    lib = load()
+    need = lib.rlic_autotune_read(None, 0)
+    buf = ctypes.create_string_buffer(int(need) + 256)
+    lib.rlic_autotune_read(buf, len(buf))
+    return json.loads(buf.value.decode())
+
+
+def autotune_clear() -> None:
+    load().rlic_autotune_clear()
+
+
+def autotune_load(path) -> int:
+    """Add the plan lines of a text file to the autotuner's table."""
+    n = load().rlic_autotune_load(str(path).encode())
+    if n < 0:
+        raise_for_status(int(n))
+    return int(n)
 
 
 def trace_stats_read() -> dict:

@@ -147,21 +147,163 @@ int env_int(const char* name, int dflt) {
   return std::atoi(s);
 }
 
+// How a call's passes 2..n replay: LDS-staged with a th x tw tile, or L2
+// gathers over a tile map (strip height `band`, wave shape).
+struct ReplayPlan {
+  int32_t lds, th, tw, band, wave_w_log2;
+};
+
+template <typename T>
+hipError_t launch_plan(const ReplayPlan& p, const T* src, T* dst, const T* kernel, int32_t klen,
+                       const rlic::FieldGeom& g, int profile, const uint4* codes,
+                       hipStream_t stream) {
+  if (p.lds)
+    return rlic::launch_replay_lds_tile<T>(src, kernel, klen, g, rlic::LdsTile{p.th, p.tw},
+                                           profile, dst, codes, stream);
+  const rlic::TileMap m = rlic::make_tile_map(g.ny, g.nx, p.band, p.wave_w_log2);
+  return rlic::launch_replay<T>(src, kernel, klen, g, m, profile, dst, codes, stream);
+}
+
+const char* plan_kernel_name(const ReplayPlan& p) { return p.lds ? "lic_replay_lds" : "lic_replay"; }
+
+// Default plan: the largest LDS tile whose box fits (RLIC_REPLAY_LDS=0: none),
+// else L2 gathers over the tracing pass's tile map.
+template <typename T>
+ReplayPlan default_plan(int32_t klen) {
+  const rlic::LdsTile t = rlic::lds_replay_tile<T>(klen);
+  if (env_int("RLIC_REPLAY_LDS", 1) != 0 && t.h > 0) return {1, t.h, t.w, 0, 0};
+  return {0, 0, 0, env_int("RLIC_TILE_BAND", 16), env_int("RLIC_WAVE_W_LOG2", 3)};
+}
+
+// Replay autotuner.  The first call with at least 3 passes on a field of at
+// least 2^18 pixels for a new (dtype, shape, kernel length, profile) times
+// its pass 2 under every candidate plan — each LDS tile whose box fits
+// (64x64, 32x64, 32x32) and L2 gathers with two strip heights — on the
+// caller's stream with HIP events (all candidates write the same bits), and
+// keeps the fastest for that key.  RLIC_AUTOTUNE=0 disables it;
+// RLIC_AUTOTUNE_CACHE=<file> loads and appends the table as text.
+struct TuneKey {
+  int32_t dtype_size, ny, nx, klen, profile;
+  bool operator<(const TuneKey& o) const { return std::memcmp(this, &o, sizeof(*this)) < 0; }
+};
+struct TuneEntry {
+  ReplayPlan plan;
+  float ms;
+};
+struct Tuner {
+  std::mutex mu;
+  std::map<TuneKey, TuneEntry> table;
+  bool loaded = false;
+};
+Tuner g_tuner;
+
+// Adds the plan lines of a text file to the table; the number of lines read,
+// or -1 when the file cannot be opened.
+int64_t tuner_read_file_locked(const char* path) {
+  FILE* f = std::fopen(path, "r");
+  if (!f) return -1;
+  TuneKey k;
+  TuneEntry e;
+  int64_t count = 0;
+  while (std::fscanf(f, " rlic-replay-plan %d %d %d %d %d %d %d %d %d %d %f", &k.dtype_size,
+                     &k.ny, &k.nx, &k.klen, &k.profile, &e.plan.lds, &e.plan.th, &e.plan.tw,
+                     &e.plan.band, &e.plan.wave_w_log2, &e.ms) == 11) {
+    g_tuner.table[k] = e;
+    ++count;
+  }
+  std::fclose(f);
+  return count;
+}
+
+void tuner_load_locked() {
+  if (g_tuner.loaded) return;
+  g_tuner.loaded = true;
+  const char* path = std::getenv("RLIC_AUTOTUNE_CACHE");
+  if (path && *path) (void)tuner_read_file_locked(path);
+}
+
+// A plan from the table is used only if it can run here.
+template <typename T>
+bool plan_usable(const ReplayPlan& p, int32_t klen) {
+  if (p.lds) {
+    const rlic::LdsTile big = rlic::lds_replay_tile<T>(klen);
+    const bool shape = (p.th == 64 && p.tw == 64) || (p.th == 32 && (p.tw == 64 || p.tw == 32));
+    if (!shape || big.h == 0) return false;
+    return p.th * p.tw <= big.h * big.w;  // boxes of smaller tiles are smaller
+  }
+  return p.band >= 1 && p.band <= 1024 && p.wave_w_log2 >= 3 && p.wave_w_log2 <= 5;
+}
+
+void tuner_store_locked(const TuneKey& k, const TuneEntry& e) {
+  g_tuner.table[k] = e;
+  const char* path = std::getenv("RLIC_AUTOTUNE_CACHE");
+  if (!path || !*path) return;
+  if (FILE* f = std::fopen(path, "a")) {
+    std::fprintf(f, "rlic-replay-plan %d %d %d %d %d %d %d %d %d %d %.6f\n", k.dtype_size, k.ny,
+                 k.nx, k.klen, k.profile, e.plan.lds, e.plan.th, e.plan.tw, e.plan.band,
+                 e.plan.wave_w_log2, e.ms);
+    std::fclose(f);
+  }
+}
+
+template <typename T>
+std::vector<ReplayPlan> replay_candidates(int32_t klen) {
+  std::vector<ReplayPlan> c;
+  const rlic::LdsTile big = rlic::lds_replay_tile<T>(klen);
+  if (env_int("RLIC_REPLAY_LDS", 1) != 0 && big.h > 0) {
+    if (big.h == 64) c.push_back({1, 64, 64, 0, 0});
+    if (big.h == 64 || big.w == 64) c.push_back({1, 32, 64, 0, 0});
+    c.push_back({1, 32, 32, 0, 0});
+  }
+  c.push_back({0, 0, 0, 16, 3});
+  c.push_back({0, 0, 0, 4, 3});
+  return c;
+}
+
+// Pass 2 under every candidate (src -> dst), timed; returns the fastest.
+template <typename T>
+hipError_t tune_replay(const std::vector<ReplayPlan>& cands, const T* src, T* dst,
+                       const T* kernel, int32_t klen, const rlic::FieldGeom& g, int profile,
+                       const uint4* codes, hipStream_t stream, TuneEntry& best) {
+  std::vector<hipEvent_t> ev(cands.size() + 1, nullptr);
+  hipError_t e = hipSuccess;
+  for (auto& x : ev)
+    if (e == hipSuccess) e = hipEventCreate(&x);
+  if (e == hipSuccess) e = hipEventRecord(ev[0], stream);
+  for (size_t k = 0; k < cands.size() && e == hipSuccess; ++k) {
+    {
+      LaunchTimer timer(plan_kernel_name(cands[k]), stream);
+      e = launch_plan<T>(cands[k], src, dst, kernel, klen, g, profile, codes, stream);
+    }
+    if (e == hipSuccess) e = hipEventRecord(ev[k + 1], stream);
+  }
+  if (e == hipSuccess) e = hipEventSynchronize(ev.back());
+  best.ms = -1.f;
+  for (size_t k = 0; k < cands.size() && e == hipSuccess; ++k) {
+    float ms = 0.f;
+    e = hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+    if (e == hipSuccess && (best.ms < 0.f || ms < best.ms)) best = {cands[k], ms};
+  }
+  for (auto& x : ev)
+    if (x) (void)hipEventDestroy(x);
+  return e;
+}
+
 // The replay passes of one call as one hipGraph.  The passes are captured
 // on a private stream (capture records, it runs nothing), instantiated, and
 // launched on the caller's stream; instantiated graphs are cached by every
 // argument of their launches (buffers included), so a call that repeats an
 // earlier one on the same buffers replays the instantiated graph.
 struct ReplayKey {
-  int dtype_size, device;
+  int32_t dtype_size, device;
   const void* a;
   const void* b;
   const void* kernel;
   const void* codes;
   int32_t klen;
   rlic::FieldGeom g;
-  rlic::TileMap m;
-  int profile, use_lds;
+  ReplayPlan plan;
+  int32_t profile, pad;  // pad: no implicit padding bytes (compared with memcmp)
   int64_t iterations;
   bool operator<(const ReplayKey& o) const { return std::memcmp(this, &o, sizeof(*this)) < 0; }
 };
@@ -176,17 +318,14 @@ struct GraphCache {
 GraphCache g_graphs;
 
 template <typename T>
-hipError_t launch_replays(const T* kernel, int32_t klen, const rlic::FieldGeom& g,
-                          const rlic::TileMap& m, int profile, bool use_lds, T* a, T* b,
-                          const uint4* codes, int64_t iterations, hipStream_t stream) {
+hipError_t launch_replays(const ReplayPlan& plan, const T* kernel, int32_t klen,
+                          const rlic::FieldGeom& g, int profile, T* a, T* b, const uint4* codes,
+                          int64_t first, int64_t iterations, hipStream_t stream) {
   // pass `it` (1-based) reads A when even, B when odd, and writes the other
-  for (int64_t it = 2; it <= iterations; ++it) {
+  for (int64_t it = first; it <= iterations; ++it) {
     const T* src = (it % 2 == 0) ? a : b;
     T* dst = (it % 2 == 0) ? b : a;
-    hipError_t e = use_lds ? rlic::launch_replay_lds<T>(src, kernel, klen, g, profile, dst, codes,
-                                                         stream)
-                           : rlic::launch_replay<T>(src, kernel, klen, g, m, profile, dst, codes,
-                                                    stream);
+    const hipError_t e = launch_plan<T>(plan, src, dst, kernel, klen, g, profile, codes, stream);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
@@ -195,9 +334,9 @@ hipError_t launch_replays(const T* kernel, int32_t klen, const rlic::FieldGeom& 
 // Runs the replay passes as a (cached) graph; false when graphs are not
 // available here (the caller then launches the passes directly).
 template <typename T>
-bool replay_graph(int device, const T* kernel, int32_t klen, const rlic::FieldGeom& g,
-                  const rlic::TileMap& m, int profile, bool use_lds, T* a, T* b,
-                  const uint4* codes, int64_t iterations, hipStream_t stream) {
+bool replay_graph(int device, const ReplayPlan& plan, const T* kernel, int32_t klen,
+                  const rlic::FieldGeom& g, int profile, T* a, T* b, const uint4* codes,
+                  int64_t iterations, hipStream_t stream) {
   if (device < 0 || device >= 256) return false;
   ReplayKey key;
   std::memset(&key, 0, sizeof(key));
@@ -209,9 +348,8 @@ bool replay_graph(int device, const T* kernel, int32_t klen, const rlic::FieldGe
   key.codes = codes;
   key.klen = klen;
   key.g = g;
-  key.m = m;
+  key.plan = plan;
   key.profile = profile;
-  key.use_lds = use_lds;
   key.iterations = iterations;
   std::lock_guard<std::mutex> lock(g_graphs.mu);
   hipGraphExec_t exec = nullptr;
@@ -231,7 +369,7 @@ bool replay_graph(int device, const T* kernel, int32_t klen, const rlic::FieldGe
       (void)hipGetLastError();
       return false;
     }
-    const hipError_t le = launch_replays<T>(kernel, klen, g, m, profile, use_lds, a, b, codes,
+    const hipError_t le = launch_replays<T>(plan, kernel, klen, g, profile, a, b, codes, 2,
                                             iterations, cap);
     const hipError_t ce = hipStreamEndCapture(cap, &graph);
     if (le != hipSuccess || ce != hipSuccess || !graph ||
@@ -254,7 +392,7 @@ bool replay_graph(int device, const T* kernel, int32_t klen, const rlic::FieldGe
     }
     g_graphs.entries.emplace(key, std::make_pair(exec, ++g_graphs.clock));
   }
-  LaunchTimer timer(use_lds ? "lic_replay_lds" : "lic_replay", stream, iterations - 1);
+  LaunchTimer timer(plan_kernel_name(plan), stream, iterations - 1);
   if (hipGraphLaunch(exec, stream) != hipSuccess) {
     (void)hipGetLastError();
     return false;
@@ -330,8 +468,6 @@ int run_on_device(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
   }
   if (replay) {
     auto* codes = static_cast<uint4*>(code_mem.ptr);
-    const int lds_policy = env_int("RLIC_REPLAY_LDS", 1);  // 0 off, 1 when it fits
-    const bool use_lds = lds_policy != 0 && rlic::lds_replay_fits<T>(klen);
     // pass `it` (1-based) writes A when odd, B when even; the last lands in out
     T* a = ((iterations - 1) % 2 == 0) ? out : tmp;
     T* b = (a == out) ? tmp : out;
@@ -357,25 +493,37 @@ int run_on_device(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
       g_stats_executed += static_cast<int64_t>(host_st[1]);
     }
     if (iterations == 1) return RLIC_OK;  // recorded only for the statistics
-    // passes 2..n: direct launches, or one captured graph with RLIC_GRAPH=1
-    // (measured no faster on MI355X: kept as an option)
-    if (env_int("RLIC_GRAPH", 0) != 0 &&
-        replay_graph<T>(dev, kernel, klen, g, m, profile, use_lds, a, b, codes, iterations,
-                        stream))
+    // the replay plan: tuned (see Tuner), or the default
+    ReplayPlan plan = default_plan<T>(klen);
+    int64_t first = 2;
+    if (env_int("RLIC_AUTOTUNE", 1) != 0 && iterations >= 3 && n >= (int64_t{1} << 18)) {
+      const TuneKey key{static_cast<int32_t>(sizeof(T)), ny, nx, klen, profile};
+      std::lock_guard<std::mutex> lock(g_tuner.mu);
+      tuner_load_locked();
+      auto hit = g_tuner.table.find(key);
+      if (hit != g_tuner.table.end()) {
+        if (plan_usable<T>(hit->second.plan, klen)) plan = hit->second.plan;
+      } else {
+        const std::vector<ReplayPlan> cands = replay_candidates<T>(klen);
+        TuneEntry best{plan, -1.f};
+        RLIC_HIP_TRY(tune_replay<T>(cands, a, b, kernel, klen, g, profile, codes, stream, best),
+                     "tuning the replay");
+        tuner_store_locked(key, best);
+        plan = best.plan;
+        first = 3;  // pass 2 is done (every candidate wrote the same bits)
+      }
+    }
+    // passes first..n: direct launches, or one captured graph with
+    // RLIC_GRAPH=1 (measured no faster on MI355X: kept as an option)
+    if (first == 2 && env_int("RLIC_GRAPH", 0) != 0 &&
+        replay_graph<T>(dev, plan, kernel, klen, g, profile, a, b, codes, iterations, stream))
       return RLIC_OK;
-    for (int64_t it = 2; it <= iterations; ++it) {
+    for (int64_t it = first; it <= iterations; ++it) {
       const T* src = (it % 2 == 0) ? a : b;
       T* dst = (it % 2 == 0) ? b : a;
-      if (use_lds) {
-        LaunchTimer timer("lic_replay_lds", stream);
-        RLIC_HIP_TRY(
-            rlic::launch_replay_lds<T>(src, kernel, klen, g, profile, dst, codes, stream),
-            "replay");
-      } else {
-        LaunchTimer timer("lic_replay", stream);
-        RLIC_HIP_TRY(rlic::launch_replay<T>(src, kernel, klen, g, m, profile, dst, codes, stream),
-                     "replay");
-      }
+      LaunchTimer timer(plan_kernel_name(plan), stream);
+      RLIC_HIP_TRY(launch_plan<T>(plan, src, dst, kernel, klen, g, profile, codes, stream),
+                   "replay");
     }
     return RLIC_OK;
   }
@@ -760,6 +908,47 @@ int64_t rlic_profiler_read(char* buf, int64_t len) {
     g_prof_done.clear();  // handed over: start a new record
   }
   return need;
+}
+
+int64_t rlic_autotune_read(char* buf, int64_t len) {
+  std::lock_guard<std::mutex> lock(g_tuner.mu);
+  tuner_load_locked();
+  std::string json = "[";
+  bool first = true;
+  for (const auto& kv : g_tuner.table) {
+    const TuneKey& k = kv.first;
+    const TuneEntry& e = kv.second;
+    char item[320];
+    snprintf(item, sizeof(item),
+             "%s{\"dtype_size\": %d, \"ny\": %d, \"nx\": %d, \"kernel_len\": %d, \"profile\": %d, "
+             "\"replay\": \"%s\", \"tile\": [%d, %d], \"band\": %d, \"wave_w_log2\": %d, "
+             "\"pass_ms\": %.6f}",
+             first ? "" : ", ", k.dtype_size, k.ny, k.nx, k.klen, k.profile,
+             e.plan.lds ? "lds" : "l2", e.plan.th, e.plan.tw, e.plan.band, e.plan.wave_w_log2,
+             e.ms);
+    json += item;
+    first = false;
+  }
+  json += "]";
+  const int64_t need = static_cast<int64_t>(json.size()) + 1;
+  if (buf && len >= need) std::memcpy(buf, json.c_str(), need);
+  return need;
+}
+
+int64_t rlic_autotune_load(const char* path) {
+  t_last_error.clear();
+  if (!path) return set_error(RLIC_ERR_INVALID_ARGUMENT, "null path");
+  std::lock_guard<std::mutex> lock(g_tuner.mu);
+  const int64_t n = tuner_read_file_locked(path);
+  if (n < 0) return set_error(RLIC_ERR_INVALID_ARGUMENT, "cannot open %s", path);
+  return n;
+}
+
+int rlic_autotune_clear(void) {
+  std::lock_guard<std::mutex> lock(g_tuner.mu);
+  g_tuner.table.clear();
+  g_tuner.loaded = true;  // a cleared table does not reload the cache file
+  return RLIC_OK;
 }
 
 int rlic_trace_stats_read(int64_t* live_lane_steps, int64_t* executed_lane_steps) {

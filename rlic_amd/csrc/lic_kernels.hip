@@ -577,6 +577,34 @@ hipError_t launch_replay_lds(const T* x, const T* kernel, int32_t klen, const Fi
   return hipGetLastError();
 }
 
+template <typename T, bool FMA>
+static bool replay_lds_tile_fma(const T* x, const T* kernel, int32_t klen, const FieldGeom& g,
+                                LdsTile tile, T* out, const uint4* codes, hipStream_t stream) {
+  if (tile.h == 64 && tile.w == 64) {
+    replay_lds_one<T, FMA, 64, 64>(x, kernel, klen, g, out, codes, stream);
+  } else if (tile.h == 32 && tile.w == 64) {
+    replay_lds_one<T, FMA, 32, 64>(x, kernel, klen, g, out, codes, stream);
+  } else if (tile.h == 32 && tile.w == 32) {
+    replay_lds_one<T, FMA, 32, 32>(x, kernel, klen, g, out, codes, stream);
+  } else {
+    return false;
+  }
+  return true;
+}
+
+template <typename T>
+hipError_t launch_replay_lds_tile(const T* x, const T* kernel, int32_t klen, const FieldGeom& g,
+                                  LdsTile tile, int profile, T* out, const uint4* codes,
+                                  hipStream_t stream) {
+  if (lds_replay_bytes(tile.h, tile.w, klen, sizeof(T)) + 2048 >
+      static_cast<size_t>(lds_replay_limit_bytes()))
+    return hipErrorInvalidValue;
+  const bool ok = (profile & 1)
+                      ? replay_lds_tile_fma<T, true>(x, kernel, klen, g, tile, out, codes, stream)
+                      : replay_lds_tile_fma<T, false>(x, kernel, klen, g, tile, out, codes, stream);
+  return ok ? hipGetLastError() : hipErrorInvalidValue;
+}
+
 #define RLIC_INSTANTIATE(T)                                                                     \
   template hipError_t launch_pack<T>(const T*, int64_t, int64_t, const T*, int64_t, int64_t,   \
                                      const T*, int64_t, int64_t, int32_t, int32_t, Cell<T>*,   \
@@ -592,7 +620,9 @@ hipError_t launch_replay_lds(const T* x, const T* kernel, int32_t klen, const Fi
   template bool lds_replay_fits<T>(int32_t);                                                   \
   template LdsTile lds_replay_tile<T>(int32_t);                                                \
   template hipError_t launch_replay_lds<T>(const T*, const T*, int32_t, const FieldGeom&, int, \
-                                           T*, const uint4*, hipStream_t);
+                                           T*, const uint4*, hipStream_t);                      \
+  template hipError_t launch_replay_lds_tile<T>(const T*, const T*, int32_t, const FieldGeom&,  \
+                                                LdsTile, int, T*, const uint4*, hipStream_t);
 
 RLIC_INSTANTIATE(float)
 RLIC_INSTANTIATE(double)

@@ -67,4 +67,11 @@ template <typename T>
 hipError_t launch_replay_lds(const T* x, const T* kernel, int32_t klen, const FieldGeom& g,
                              int profile, T* out, const uint4* codes, hipStream_t stream);
 
+// The same with an explicit tile (64x64, 32x64 or 32x32); hipErrorInvalidValue
+// when that tile's box does not fit.
+template <typename T>
+hipError_t launch_replay_lds_tile(const T* x, const T* kernel, int32_t klen, const FieldGeom& g,
+                                  LdsTile tile, int profile, T* out, const uint4* codes,
+                                  hipStream_t stream);
+
 }  // namespace rlic
