@@ -26,6 +26,7 @@
 #include "../../include/rlic_hip.h"
 #include "lic_batch.h"
 #include "lic_launch.h"
+#include "lic_scratch.h"
 #include "lic_serve.h"
 
 namespace {
@@ -416,19 +417,43 @@ int run_on_device(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
     (void)hipGetLastError();
     dev = -1;
   }
-  // kernel weights with zero padding on both sides
-  DevMem kpad_mem;
   const int32_t pad = rlic::kernel_pad();
-  RLIC_HIP_TRY(kpad_mem.alloc(sizeof(T) * (klen + 2 * pad), stream), "allocating kernel copy");
+  const bool stats = env_int("RLIC_TRACE_STATS", 0) != 0;
+  const bool need_tmp = iterations > 1 || stats;
+  // Move record: code_chunks(klen) 16-byte chunks per pixel.
+  const size_t code_bytes =
+      sizeof(uint4) * static_cast<size_t>(rlic::code_chunks(klen)) * static_cast<size_t>(n);
+  bool replay = need_tmp && env_int("RLIC_REPLAY", 1) != 0;
+  if (replay && code_bytes > (size_t{256} << 20)) {
+    // a large record must leave half of the free memory; small ones skip
+    // the (slow) query and fall back below if the allocation fails
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+    replay = code_bytes <= free_b / 2;
+  }
+  // one scratch block: padded weights | cells | ping-pong buffer | move record
+  auto up = [](size_t x) { return (x + 255) & ~size_t{255}; };
+  const size_t off_cells = up(sizeof(T) * static_cast<size_t>(klen + 2 * pad));
+  const size_t off_tmp = off_cells + up(sizeof(rlic::Cell<T>) * static_cast<size_t>(n));
+  const size_t off_codes = off_tmp + (need_tmp ? up(sizeof(T) * static_cast<size_t>(n)) : 0);
+  rlic::CallScratch scratch;
+  hipError_t ae = scratch.get(off_codes + (replay ? code_bytes : 0), stream);
+  if (ae != hipSuccess && replay) {  // no room for the record: trace every pass
+    (void)hipGetLastError();
+    replay = false;
+    ae = scratch.get(off_codes, stream);
+  }
+  RLIC_HIP_TRY(ae, "allocating call scratch");
+  char* base = static_cast<char*>(scratch.ptr());
+  T* kpad = reinterpret_cast<T*>(base);
+  auto* c = reinterpret_cast<rlic::Cell<T>*>(base + off_cells);
+  T* tmp = need_tmp ? reinterpret_cast<T*>(base + off_tmp) : nullptr;
+  uint4* code_ptr = replay ? reinterpret_cast<uint4*>(base + off_codes) : nullptr;
   {
     LaunchTimer timer("lic_pad_kernel", stream);
-    RLIC_HIP_TRY(rlic::launch_pad_kernel<T>(kernel, klen, static_cast<T*>(kpad_mem.ptr), stream),
-                 "kernel copy");
+    RLIC_HIP_TRY(rlic::launch_pad_kernel<T>(kernel, klen, kpad, stream), "kernel copy");
   }
-  kernel = static_cast<const T*>(kpad_mem.ptr) + pad;
-  DevMem cells;
-  RLIC_HIP_TRY(cells.alloc(sizeof(rlic::Cell<T>) * n, stream), "allocating packed cells");
-  auto* c = static_cast<rlic::Cell<T>*>(cells.ptr);
+  kernel = kpad + pad;
   {
     LaunchTimer timer("lic_pack", stream);
     RLIC_HIP_TRY(
@@ -437,7 +462,6 @@ int run_on_device(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
   }
   const rlic::TileMap m = rlic::make_tile_map(ny, nx, env_int("RLIC_TILE_BAND", 16),
                                               env_int("RLIC_WAVE_W_LOG2", 3));
-  const bool stats = env_int("RLIC_TRACE_STATS", 0) != 0;
   if (iterations == 1 && !stats) {
     LaunchTimer timer("lic_trace", stream);
     RLIC_HIP_TRY(rlic::launch_trace<T>(c, kernel, klen, g, m, uv_mode, profile, out, nullptr,
@@ -445,29 +469,8 @@ int run_on_device(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
                  "trace");
     return RLIC_OK;
   }
-  DevMem tmp_mem;
-  RLIC_HIP_TRY(tmp_mem.alloc(sizeof(T) * n, stream), "allocating iteration buffer");
-  T* tmp = static_cast<T*>(tmp_mem.ptr);
-
-  // Move record: code_chunks(klen) 16-byte chunks per pixel.
-  const size_t code_bytes =
-      sizeof(uint4) * static_cast<size_t>(rlic::code_chunks(klen)) * static_cast<size_t>(n);
-  bool replay = env_int("RLIC_REPLAY", 1) != 0;
-  if (replay && code_bytes > (size_t{256} << 20)) {
-    // a large record must leave half of the free memory; small ones skip
-    // the (slow) query and fall back below if the allocation fails
-    size_t free_b = 0, total_b = 0;
-    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
-    replay = code_bytes <= free_b / 2;
-  }
-  DevMem code_mem;
-  if (replay && code_mem.alloc(code_bytes, stream) != hipSuccess) {
-    (void)hipGetLastError();
-    code_mem.ptr = nullptr;
-    replay = false;
-  }
   if (replay) {
-    auto* codes = static_cast<uint4*>(code_mem.ptr);
+    uint4* codes = code_ptr;
     // pass `it` (1-based) writes A when odd, B when even; the last lands in out
     T* a = ((iterations - 1) % 2 == 0) ? out : tmp;
     T* b = (a == out) ? tmp : out;

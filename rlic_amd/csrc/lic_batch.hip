@@ -26,6 +26,7 @@
 #include "lic_batch.h"
 #include "lic_device.h"
 #include "lic_launch.h"
+#include "lic_scratch.h"
 #include "lic_steps.h"
 
 namespace rlic {
@@ -360,12 +361,12 @@ int launch_queue(std::vector<BatchField<T>>& fields, const std::vector<const T*>
   const size_t n_counters = segs.size() + 4;  // done per segment, queue head (8 B), failure flag
   const size_t off_pool = align_up(off_done + sizeof(uint32_t) * n_counters);
   const size_t bytes = off_pool + sizeof(T) * static_cast<size_t>(pool_len);
-  BatchScratch scratch;
-  if (hipError_t e = scratch.reserve(bytes, stream); e != hipSuccess) {
+  CallScratch scratch;  // descriptors: reused from the per-stream cache
+  if (hipError_t e = scratch.get(bytes, stream); e != hipSuccess) {
     err = std::string("allocating queue descriptors: ") + hipGetErrorString(e);
     return e == hipErrorOutOfMemory ? RLIC_ERR_OUT_OF_MEMORY : RLIC_ERR_HIP;
   }
-  char* base = static_cast<char*>(scratch.ptr);
+  char* base = static_cast<char*>(scratch.ptr());
   T* d_pool = reinterpret_cast<T*>(base + off_pool);
   for (BatchField<T>& f : fields) f.kernel = d_pool + f.kernel_off;
   std::vector<char> host(off_done);
