@@ -79,15 +79,19 @@ def cpu_baseline(wl, rows: int) -> dict:
 
 
 def bench_batch(args, dev) -> int:
-    """512 uneven fields: one persistent batched launch vs a per-call loop."""
+    """512 uneven fields (sides 128..2048, or 32..256 for batch512small): one
+    persistent batched launch vs a per-call loop."""
     import torch
 
     from rlic_amd import _core, _native
     from rlic_amd.batch import convolve_batch
     from rlic_amd.workloads import batch_uneven, pixel_steps
 
-    log("bench: generating 512 uneven fields on the device")
-    fields = batch_uneven(512, seed=4, device=dev)
+    small = args.workload == "batch512small"
+    log(f"bench: generating 512 uneven fields on the device (sides "
+        f"{'32..256' if small else '128..2048'})")
+    fields = (batch_uneven(512, seed=4, device=dev, min_side=32, max_side=256) if small
+              else batch_uneven(512, seed=4, device=dev))
     steps_total = pixel_steps(fields)
     bpairs = []
     for f in fields:
@@ -132,7 +136,8 @@ def bench_batch(args, dev) -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "batch_uneven_512_f32", "fields": len(fields),
+        "config": {"workload": "batch_uneven_512_f32" + ("_small" if small else ""),
+                   "fields": len(fields), "sides": "32..256" if small else "128..2048",
                    "pixel_steps_per_step": steps_total},
         "per_call": {"value": round(per_call_value, 3),
                      "ms_per_step": round(timings["per_call"] * 1e3, 3)},
@@ -292,7 +297,7 @@ def main() -> int:
         log(f"bench: {args.calibrate_copy} calibration copies of 1 GiB done")
         del src, dst
         torch.cuda.empty_cache()
-    if args.workload == "batch512":
+    if args.workload in ("batch512", "batch512small"):
         return bench_batch(args, dev)
     if args.workload == "serve":
         return bench_serve(args, dev)

@@ -45,15 +45,21 @@ def _is_torch(x: Any) -> bool:
     return type(x).__module__.split(".", 1)[0] == "torch"
 
 
+_TORCH_DTYPES: dict = {}
+
+
 def _np_dtype(arr: Any) -> Any:
     """The array's dtype as a NumPy dtype (torch dtypes mapped when possible)."""
     if _is_torch(arr):
-        import torch
+        dt = arr.dtype
+        if dt not in _TORCH_DTYPES:
+            import torch
 
-        try:
-            return np.dtype(torch.empty((), dtype=arr.dtype).numpy().dtype)
-        except TypeError:  # e.g. bfloat16: no NumPy equivalent
-            return arr.dtype
+            try:
+                _TORCH_DTYPES[dt] = np.dtype(torch.empty((), dtype=dt).numpy().dtype)
+            except TypeError:  # e.g. bfloat16: no NumPy equivalent
+                _TORCH_DTYPES[dt] = dt
+        return _TORCH_DTYPES[dt]
     return arr.dtype
 
 
@@ -189,19 +195,20 @@ def validate(texture, u, v, *, kernel, uv_mode, boundaries, iterations, check_va
         )
 
     dtypes = [_np_dtype(a) for a in (texture, u, v, kernel)]
-    expectation = (
-        "Expected texture, u, v and kernel with identical dtype, from "
-        f"{_SUPPORTED_DTYPES}. Got texture.dtype={dtypes[0]!r}, u.dtype={dtypes[1]!r}, "
-        f"v.dtype={dtypes[2]!r}, kernel.dtype={dtypes[3]!r}"
-    )
     distinct = list(dict.fromkeys(dtypes))
     unsupported = [dt for dt in distinct if dt not in _SUPPORTED_DTYPES]
-    if unsupported:
-        problems.append(
-            TypeError(f"Found unsupported data type(s): {unsupported}. {expectation}")
+    if unsupported or len(distinct) != 1:
+        expectation = (
+            "Expected texture, u, v and kernel with identical dtype, from "
+            f"{_SUPPORTED_DTYPES}. Got texture.dtype={dtypes[0]!r}, u.dtype={dtypes[1]!r}, "
+            f"v.dtype={dtypes[2]!r}, kernel.dtype={dtypes[3]!r}"
         )
-    if len(distinct) != 1:
-        problems.append(TypeError(f"Data types mismatch. {expectation}"))
+        if unsupported:
+            problems.append(
+                TypeError(f"Found unsupported data type(s): {unsupported}. {expectation}")
+            )
+        if len(distinct) != 1:
+            problems.append(TypeError(f"Data types mismatch. {expectation}"))
 
     if texture.ndim != 2:
         problems.append(
