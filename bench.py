@@ -67,6 +67,14 @@ def cpu_baseline(wl, rows: int) -> dict:
                  boundaries=wl.boundaries, iterations=1, rows=(r0, r1), threads=1)
     dt = time.perf_counter() - t0
     steps = (r1 - r0) * nx * (wl.kernel.size - 1)
+    # the same band with OpenMP over rows on the host's cores (at most 16:
+    # one GPU's share of the host), a parallel extension of the reference's
+    # single-threaded loop
+    threads = max(1, min(16, cpu.max_threads()))
+    t0 = time.perf_counter()
+    cpu.convolve(wl.texture, wl.u, wl.v, kernel=wl.kernel, uv_mode=wl.uv_mode,
+                 boundaries=wl.boundaries, iterations=1, rows=(r0, r1), threads=threads)
+    dt_par = time.perf_counter() - t0
     return {
         "value": steps / dt / 1e9,
         "unit": "Gpixel-steps/s",
@@ -75,6 +83,8 @@ def cpu_baseline(wl, rows: int) -> dict:
         "sample": f"pass 1 of rows [{r0},{r1}) x {nx} cols "
                   f"({steps / 1e6:.1f} M pixel-steps, {dt:.2f} s), single thread, "
                   f"host CPU: {os.cpu_count()} logical cores visible",
+        "parallel": {"value": steps / dt_par / 1e9, "cores": threads,
+                     "sample": f"the same rows, OpenMP over rows, {dt_par:.2f} s"},
     }
 
 

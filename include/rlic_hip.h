@@ -222,6 +222,12 @@ int64_t rlic_autotune_read(char* buf, int64_t len);
 int64_t rlic_autotune_load(const char* path);
 int rlic_autotune_clear(void);
 
+/* Check of the float division the streamline kernels use for step times
+ * (src/lib.rs:157-180 divide in IEEE arithmetic): q[i] = a[i] / b[i] for n
+ * device-memory elements, computed on the device; synchronous.  No
+ * counterpart in the reference; for tests. */
+int rlic_div_check_f32(const float* a, const float* b, float* q, int64_t n, void* stream);
+
 /* Library version string.  No counterpart in the reference. */
 const char* rlic_version(void);
 

@@ -913,6 +913,16 @@ int64_t rlic_profiler_read(char* buf, int64_t len) {
   return need;
 }
 
+int rlic_div_check_f32(const float* a, const float* b, float* q, int64_t n, void* stream) {
+  t_last_error.clear();
+  if (n < 0 || (n > 0 && (!a || !b || !q)))
+    return set_error(RLIC_ERR_INVALID_ARGUMENT, "invalid argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  RLIC_HIP_TRY(rlic::launch_div_check(a, b, q, n, s), "division check");
+  RLIC_HIP_TRY(hipStreamSynchronize(s), "division check");
+  return RLIC_OK;
+}
+
 int64_t rlic_autotune_read(char* buf, int64_t len) {
   std::lock_guard<std::mutex> lock(g_tuner.mu);
   tuner_load_locked();

@@ -501,6 +501,21 @@ hipError_t launch_replay(const T* x, const T* kernel, int32_t klen, const FieldG
   return hipGetLastError();
 }
 
+__global__ __launch_bounds__(256) void lic_div_check(const float* __restrict__ a,
+                                                     const float* __restrict__ b,
+                                                     float* __restrict__ q, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (i < n) q[i] = div_rn(a[i], b[i]);
+}
+
+hipError_t launch_div_check(const float* a, const float* b, float* q, int64_t n,
+                            hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(lic_div_check, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0,
+                     stream, a, b, q, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_record_stats(const uint4* codes, int32_t klen, const FieldGeom& g,
                                const TileMap& m, unsigned long long* stats, hipStream_t stream) {
   hipLaunchKernelGGL(lic_record_stats, dim3(m.n_tiles), dim3(kBlock), 0, stream, codes, klen, g,

@@ -48,6 +48,10 @@ hipError_t launch_replay(const T* x, const T* kernel, int32_t klen, const FieldG
                          const TileMap& m, int profile, T* out, const uint4* codes,
                          hipStream_t stream);
 
+// q[i] = a[i] / b[i] with the streamline kernels' float division (a check).
+hipError_t launch_div_check(const float* a, const float* b, float* q, int64_t n,
+                            hipStream_t stream);
+
 // Adds the live lane-steps (stats[0]) and executed lane-steps (stats[1]) of
 // a tracing pass with tile map m, read from its move record.
 hipError_t launch_record_stats(const uint4* codes, int32_t klen, const FieldGeom& g,
