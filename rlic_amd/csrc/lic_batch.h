@@ -29,9 +29,6 @@ struct BatchScratch {
   }
 };
 
-// Pixels per side of one work item (a square region of one field).
-constexpr int32_t kRegion = 32;
-
 // One field of a queue launch, as the device reads it.
 template <typename T>
 struct BatchField {

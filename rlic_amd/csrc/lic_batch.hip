@@ -2,7 +2,7 @@
 //
 // Each field is one `rlic.convolve` call (src/lib.rs:408-443) with its own
 // size, kernel, boundaries, uv_mode and iteration count.  Work items are
-// 32 x 32 pixel regions of one pass of one field; a segment is the run of
+// square pixel regions of one pass of one field; a segment is the run of
 // items of one (field, pass).  The waves of a persistent grid pop items from
 // one atomic counter.  Pass 0 traces the streamlines from the packed cells
 // (and records the moves when more passes follow); later passes replay the
@@ -124,15 +124,17 @@ __device__ __forceinline__ void batch_replay_pixel(const BatchField<T>& f,
   dst[idx0] = acc;
 }
 
-// The 32 x 32 region of a work item as 16 squares of 8 x 8 lanes; (i_base,
-// j_base) is the lane's pixel in the first square.
+// The region of a work item, (8 << sq_log2) pixels on a side, as squares of
+// 8 x 8 lanes; (i_base, j_base) is the lane's pixel in the first square.
 template <typename T, bool FMA, bool BRANCHLESS, int POL, int REC, bool BUF>
 __device__ __forceinline__ void batch_trace_region(const BatchField<T>& f,
                                                    const T* __restrict__ kernel, T* dst,
-                                                   int32_t i_base, int32_t j_base) {
+                                                   int32_t i_base, int32_t j_base,
+                                                   int32_t sq_log2) {
+  const int32_t sq_mask = (1 << sq_log2) - 1;
 #pragma unroll 1
-  for (int32_t st = 0; st < 16; ++st) {
-    const int32_t i0 = i_base + (st >> 2) * 8, j0 = j_base + (st & 3) * 8;
+  for (int32_t st = 0; st < (1 << (2 * sq_log2)); ++st) {
+    const int32_t i0 = i_base + (st >> sq_log2) * 8, j0 = j_base + (st & sq_mask) * 8;
     if (i0 < f.g.ny && j0 < f.g.nx)
       batch_trace_pixel<T, FMA, BRANCHLESS, POL, REC, BUF>(f, kernel, dst, i0, j0);
   }
@@ -142,10 +144,12 @@ template <typename T, bool FMA, bool BUF>
 __device__ __forceinline__ void batch_replay_region(const BatchField<T>& f,
                                                     const T* __restrict__ kernel, const T* src,
                                                     T* dst, const uint32_t* delta,
-                                                    int32_t i_base, int32_t j_base) {
+                                                    int32_t i_base, int32_t j_base,
+                                                    int32_t sq_log2) {
+  const int32_t sq_mask = (1 << sq_log2) - 1;
 #pragma unroll 1
-  for (int32_t st = 0; st < 16; ++st) {
-    const int32_t i0 = i_base + (st >> 2) * 8, j0 = j_base + (st & 3) * 8;
+  for (int32_t st = 0; st < (1 << (2 * sq_log2)); ++st) {
+    const int32_t i0 = i_base + (st >> sq_log2) * 8, j0 = j_base + (st & sq_mask) * 8;
     if (i0 < f.g.ny && j0 < f.g.nx)
       batch_replay_pixel<T, FMA, BUF>(f, kernel, src, dst, delta, i0, j0);
   }
@@ -160,11 +164,12 @@ __device__ __forceinline__ int64_t lane0_value(uint64_t x) {
 }
 
 // Persistent kernel: every wave is an independent worker.  A wave pops one
-// work item (a 32 x 32 region of one pass of one field), waits if needed
-// (DEPS) for the segment it depends on, computes the region's pixels as 16
-// squares of 8 x 8 lanes, then (DEPS) counts the item done.  There is no
-// block barrier in the loop, so waves of one block never wait for each
-// other, and every branch that steers the loop is wave-uniform.
+// work item (a square region of one pass of one field), waits if needed
+// (DEPS) for the segment it depends on, computes the region's pixels as
+// squares of 8 x 8 lanes, then (DEPS) counts the item done.  Regions are
+// 8 << sq_log2 pixels on a side (16, 32 or 64).  There is no block barrier
+// in the loop, so waves of one block never wait for each other, and every
+// branch that steers the loop is wave-uniform.
 template <typename T, bool FMA, bool BRANCHLESS, bool BUF, bool DEPS>
 __global__ __launch_bounds__(256) void lic_batch_persistent(const BatchField<T>* __restrict__ fields,
                                                             const Segment* __restrict__ segs,
@@ -173,7 +178,7 @@ __global__ __launch_bounds__(256) void lic_batch_persistent(const BatchField<T>*
                                                             int64_t total_items,
                                                             unsigned long long* queue,
                                                             uint32_t* done, uint32_t* failed,
-                                                            int64_t wait_ticks) {
+                                                            int64_t wait_ticks, int32_t sq_log2) {
   __shared__ uint32_t delta_s[4][16];
   const int32_t lane = threadIdx.x & 63;
   uint32_t* delta = delta_s[threadIdx.x >> 6];
@@ -277,20 +282,24 @@ __global__ __launch_bounds__(256) void lic_batch_persistent(const BatchField<T>*
     const int64_t local = item - seg.item0;
     const int32_t ry = static_cast<int32_t>(local / f.regions_x);
     const int32_t rx = static_cast<int32_t>(local - static_cast<int64_t>(ry) * f.regions_x);
-    const int32_t i_base = ry * kRegion + py, j_base = rx * kRegion + px;
+    const int32_t i_base = (ry << (sq_log2 + 3)) + py, j_base = (rx << (sq_log2 + 3)) + px;
     const T* __restrict__ kern = kpool + f.kernel_off;  // uniform: weights via scalar loads
     T* dst = static_cast<T*>(seg.dst);
     if (seg.pass == 0) {
       // one specialised step loop per (uv_mode, recording) pair, each with
       // its own loop over the region (the branches are uniform)
       const bool pol = f.uv_mode == 1, rec = seg.record != 0;
-      if (pol && rec) batch_trace_region<T, FMA, BRANCHLESS, 1, 1, BUF>(f, kern, dst, i_base, j_base);
-      else if (pol) batch_trace_region<T, FMA, BRANCHLESS, 1, 0, BUF>(f, kern, dst, i_base, j_base);
-      else if (rec) batch_trace_region<T, FMA, BRANCHLESS, 0, 1, BUF>(f, kern, dst, i_base, j_base);
-      else batch_trace_region<T, FMA, BRANCHLESS, 0, 0, BUF>(f, kern, dst, i_base, j_base);
+      if (pol && rec) batch_trace_region<T, FMA, BRANCHLESS, 1, 1, BUF>(f, kern, dst, i_base,
+                                                                           j_base, sq_log2);
+      else if (pol) batch_trace_region<T, FMA, BRANCHLESS, 1, 0, BUF>(f, kern, dst, i_base,
+                                                                         j_base, sq_log2);
+      else if (rec) batch_trace_region<T, FMA, BRANCHLESS, 0, 1, BUF>(f, kern, dst, i_base,
+                                                                         j_base, sq_log2);
+      else batch_trace_region<T, FMA, BRANCHLESS, 0, 0, BUF>(f, kern, dst, i_base,
+                                                                j_base, sq_log2);
     } else {
       batch_replay_region<T, FMA, BUF>(f, kern, static_cast<const T*>(seg.src), dst, delta,
-                                       i_base, j_base);
+                                       i_base, j_base, sq_log2);
     }
     if (DEPS) note_done(lo);
   }
@@ -329,18 +338,31 @@ static int persistent_grid(const void* fn) {
 
 static size_t align_up(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
 
+// Work-item region side: RLIC_BATCH_REGION (16, 32 or 64), default 32.
+// Measured on the 512-field batches: 16 and 32 tie on small fields (sides
+// 32..256), 32 is 4% faster than 16 on large ones (sides 128..2048), 64 is
+// slower on both.
+template <typename T>
+static int32_t region_sq_log2(const std::vector<BatchField<T>>&) {
+  const char* env = std::getenv("RLIC_BATCH_REGION");
+  const int side = (env && *env) ? std::atoi(env) : 32;
+  return side == 16 ? 1 : side == 64 ? 3 : 2;
+}
+
 template <typename T>
 int launch_queue(std::vector<BatchField<T>>& fields, const std::vector<const T*>& kernel_src,
                  std::vector<Segment>& segs, bool deps, bool pack, int profile,
                  hipStream_t stream, std::string& err) {
   const size_t nfields = fields.size();
   if (nfields == 0) return RLIC_OK;
+  const int32_t sq_log2 = region_sq_log2(fields);
+  const int32_t region = 8 << sq_log2;
   // per-field regions, pack blocks and weight-pool slots
   int64_t pack_blocks = 0, pool_len = 0;
   bool buf = true;
   for (BatchField<T>& f : fields) {
-    f.regions_x = (f.g.nx + kRegion - 1) / kRegion;
-    f.regions = f.regions_x * ((f.g.ny + kRegion - 1) / kRegion);
+    f.regions_x = (f.g.nx + region - 1) / region;
+    f.regions = f.regions_x * ((f.g.ny + region - 1) / region);
     f.pack_block0 = pack_blocks;
     if (pack) pack_blocks += static_cast<int64_t>((f.g.nx + 63) / 64) * ((f.g.ny + 3) / 4);
     f.kernel_off = pool_len + kKernelPad;
@@ -416,8 +438,9 @@ int launch_queue(std::vector<BatchField<T>>& fields, const std::vector<const T*>
     const char* env = std::getenv("RLIC_BATCH_WAIT_MS");
     const int64_t wait_ticks = (env ? std::max<int64_t>(0, std::atoll(env)) : 2000) * khz;
     const int32_t nsegs = static_cast<int32_t>(segs.size());
+    int32_t sq = sq_log2;
     void* args[] = {&d_fields, &d_segs, &d_pool, const_cast<int32_t*>(&nsegs), &items,
-                    &d_queue, &d_done, &d_failed, const_cast<int64_t*>(&wait_ticks)};
+                    &d_queue, &d_done, &d_failed, const_cast<int64_t*>(&wait_ticks), &sq};
     e = hipLaunchKernel(fn, dim3(grid), dim3(256), args, 0, stream);
     if (e != hipSuccess) {
       err = std::string("queue kernel: ") + hipGetErrorString(e);

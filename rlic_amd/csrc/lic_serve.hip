@@ -4,7 +4,7 @@
 // copies its kernel weights and packs (u, v, texture) into device memory
 // that stays resident until the field is released; each step then runs one
 // pass of every field that has passes left, all fields of a dtype in one
-// persistent launch over 32 x 32 regions (launch_queue without waits: the
+// persistent launch over square regions (launch_queue without waits: the
 // step boundary orders the passes).  A field's first pass traces and, when
 // it keeps a move record, records the moves; later passes (and follow-up
 // passes requested after the field finished) replay them.  The two output
