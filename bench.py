@@ -196,42 +196,41 @@ def bench_serve(args, dev) -> int:
     log(f"bench: capacity {capacity / 1e9:.1f} Gpixel-steps/s over {len(reqs)} requests; "
         f"Poisson rate {lam:.1f} requests/s (70% load)")
 
-    # Poisson arrivals; a quarter of them follow up a finished kept request
+    # Poisson arrivals injected in real time by this thread while a
+    # background thread steps the server; a quarter of the arrivals follow
+    # up a finished kept request
     n = args.serve_requests
     gaps = rng.exponential(1.0 / lam, size=n)
     offsets = np.cumsum(gaps)
     plans = [(serving_request(rng, pool), rng.random() < 0.25, rng.random() < 0.5,
               int(rng.integers(1, 21))) for _ in range(n)]
-    done: list = []
-    kept_done: list = []
+    submitted: list = []
+    kept: list = []
     with Server(budget, device=dev) as srv:
         torch.cuda.synchronize()
+        srv.start()
         t_start = time.perf_counter()
-        k = 0
-        while k < n or srv.queued or srv.active:
-            now = time.perf_counter() - t_start
-            while k < n and offsets[k] <= now:
-                r, follow, keep, more = plans[k]
-                arrival = t_start + offsets[k]
-                if follow and kept_done:
-                    base = kept_done[int(rng.integers(len(kept_done)))]
-                    kept_done.remove(base)
-                    q = srv.follow_up(base, more, arrival=arrival)
-                else:
-                    q = submit(srv, r, arrival=arrival, keep=keep)
-                if q.state == "done":
-                    done.append(q)
-                k += 1
-            if not (srv.queued or srv.active):
-                if k < n:
-                    time.sleep(max(0.0, offsets[k] - (time.perf_counter() - t_start)))
-                continue
-            for q in srv.step():
-                done.append(q)
-                if q.keep:
-                    kept_done.append(q)
+        for k in range(n):
+            delay = offsets[k] - (time.perf_counter() - t_start)
+            if delay > 0:
+                time.sleep(delay)
+            r, follow, keep, more = plans[k]
+            arrival = t_start + offsets[k]
+            ready = [q for q in kept if q.state == "done"]
+            if follow and ready:
+                base = ready[int(rng.integers(len(ready)))]
+                kept.remove(base)
+                q = srv.follow_up(base, more, arrival=arrival)
+            else:
+                q = submit(srv, r, arrival=arrival, keep=keep)
+            submitted.append(q)
+            if q.keep:
+                kept.append(q)
+        srv.run()
         t_end = time.perf_counter()
+        srv.stop()
         steps_run, evictions, readmissions = srv.steps, srv.evictions, srv.readmissions
+    done = [q for q in submitted if q.state == "done"]
     lat = np.array([q.latency for q in done]) * 1e3
     work = sum(q.pixel_steps for q in done)
     span = t_end - t_start

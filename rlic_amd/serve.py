@@ -18,6 +18,13 @@ and the result equals ``convolve`` with the total number of iterations.
 Kept fields are freed by :meth:`Server.release` or, when an admission
 needs their memory, least recently used first; a follow-up of an evicted
 field is admitted again from its original inputs (same result, more work).
+
+The caller may drive the steps itself (:meth:`Server.step`,
+:meth:`Server.run`) or let a background thread do it (:meth:`Server.start`
+... :meth:`Server.stop`): submissions from any thread are then admitted at
+the next step boundary while the current step runs on the GPU (the native
+step releases the GIL), and :meth:`Request.wait` blocks until a request is
+done.
 """
 
 from __future__ import annotations
@@ -26,6 +33,7 @@ __all__ = ["Request", "Server"]
 
 import ctypes
 import itertools
+import threading
 import time
 from collections import OrderedDict, deque
 from dataclasses import dataclass, field
@@ -64,10 +72,20 @@ class Request:
     total_iterations: int = 0  # passes of the field after this request
     pixel_steps: int = 0       # work of this request (H*W*(L-1)*iterations)
     _field: _Field | None = None
+    _event: threading.Event = field(default_factory=threading.Event, repr=False)
 
     @property
     def latency(self) -> float | None:
         return None if self.finished is None else self.finished - self.arrival
+
+    def wait(self, timeout: float | None = None) -> bool:
+        """Block until the request is done or failed; False on timeout."""
+        return self._event.wait(timeout)
+
+    def _finish(self, state: str, when: float) -> None:
+        self.state = state
+        self.finished = when
+        self._event.set()
 
 
 @dataclass(eq=False)
@@ -89,8 +107,9 @@ class _Field:
 class Server:
     """Continuous-batching LIC server on one device (see the module docstring).
 
-    Not thread-safe: one thread submits and steps (the native calls on one
-    server are serialized anyway).
+    submit, follow_up and release may be called from any thread; steps run
+    one at a time (on the background thread after start(), else on the
+    caller's).
     """
 
     def __init__(self, budget_bytes: int = 64 << 30, *, device=None, profile=None,
@@ -112,6 +131,12 @@ class Server:
         if not ptr:
             _native.raise_for_status(_native.RLIC_ERR_INVALID_ARGUMENT)
         self._srv = ctypes.c_void_p(ptr)
+        self._lock = threading.RLock()            # Python-side state
+        self._cv = threading.Condition(self._lock)
+        self._step_lock = threading.Lock()        # one step at a time
+        self._thread: threading.Thread | None = None
+        self._stopping = False
+        self.loop_error: BaseException | None = None
         self._queue: deque[Request] = deque()
         self._active: dict[int, _Field] = {}
         self._idle: OrderedDict[int, _Field] = OrderedDict()  # kept, finished; LRU first
@@ -133,10 +158,12 @@ class Server:
         req.pixel_steps = int(ny) * int(nx) * max(int(kernel.shape[0]) - 1, 0) * req.iterations
         if req.iterations == 0:  # src/rlic/_lib.py:208-209: a copy of the texture
             req.result = texture.clone() if _core._is_torch(texture) else texture.copy()
-            req.state = "done"
-            req.admitted = req.finished = self.clock()
+            req.admitted = self.clock()
+            req._finish("done", req.admitted)
             return req
-        self._queue.append(req)
+        with self._cv:
+            self._queue.append(req)
+            self._cv.notify_all()
         return req
 
     def follow_up(self, request: Request, iterations: int, *,
@@ -158,43 +185,58 @@ class Server:
             req.iterations
         if req.iterations == 0:
             req.result = request.result
-            req.state = "done"
-            req.admitted = req.finished = self.clock()
             req._field = request._field
+            req.admitted = self.clock()
+            req._finish("done", req.admitted)
             return req
-        self._queue.append(req)
+        with self._cv:
+            self._queue.append(req)
+            self._cv.notify_all()
         return req
 
     def step(self) -> list[Request]:
         """Admit what fits, run one pass of every admitted request, and
         return the requests that finished in this step."""
-        self._admit()
-        if not self._active:
-            return []
-        cap = len(self._active)
-        out = (ctypes.c_int64 * cap)()
-        n = ctypes.c_int64(0)
-        _native.raise_for_status(self._lib.rlic_server_step(self._srv, out, cap, ctypes.byref(n)))
-        self.steps += 1
-        done = []
-        now = self.clock()
-        for h in out[: n.value]:
-            fld = self._active.pop(h)
-            req = fld.request
-            req.result = self._read(fld)
-            req.finished = self.clock()
-            req.state = "done"
-            fld.request = None
-            fld.last_used = now
-            if fld.keep:
-                self._idle[h] = fld
-            else:
-                self._release_handle(h)
-            done.append(req)
-        return done
+        with self._step_lock:
+            with self._lock:
+                self._admit()
+                if not self._active:
+                    return []
+                cap = len(self._active)
+            out = (ctypes.c_int64 * cap)()
+            n = ctypes.c_int64(0)
+            # the GPU work of the step; other threads may submit meanwhile
+            _native.raise_for_status(
+                self._lib.rlic_server_step(self._srv, out, cap, ctypes.byref(n)))
+            with self._cv:
+                self.steps += 1
+                done = []
+                now = self.clock()
+                for h in out[: n.value]:
+                    fld = self._active.pop(h)
+                    req = fld.request
+                    req.result = self._read(fld)
+                    fld.request = None
+                    fld.last_used = now
+                    if fld.keep:
+                        self._idle[h] = fld
+                    else:
+                        self._release_handle(h)
+                    req._finish("done", self.clock())
+                    done.append(req)
+                self._cv.notify_all()
+            return done
 
     def run(self) -> None:
-        """Step until no request is queued or running."""
+        """Until no request is queued or running: step (or, with the
+        background thread running, wait for it)."""
+        if self._thread is not None:
+            with self._cv:
+                while (self._queue or self._active) and self.loop_error is None:
+                    self._cv.wait(0.05)
+            if self.loop_error is not None:
+                raise self.loop_error
+            return
         while self._queue or self._active:
             before = len(self._queue) + len(self._active)
             self.step()
@@ -202,14 +244,64 @@ class Server:
                 # nothing could be admitted and nothing runs: the head cannot fit
                 raise MemoryError("a queued request does not fit the serving budget")
 
+    def start(self) -> None:
+        """Run steps on a background thread whenever requests are pending."""
+        if self._thread is not None:
+            return
+        self._stopping = False
+        self._thread = threading.Thread(target=self._loop, name="rlic-serve", daemon=True)
+        self._thread.start()
+
+    def stop(self) -> None:
+        """Finish the pending requests, then end the background thread."""
+        th = self._thread
+        if th is None:
+            return
+        with self._cv:
+            self._stopping = True
+            self._cv.notify_all()
+        th.join()
+        self._thread = None
+
+    def _loop(self) -> None:
+        while True:
+            with self._cv:
+                while not (self._queue or self._active):
+                    if self._stopping:
+                        return
+                    self._cv.wait()
+            try:
+                before = (len(self._queue), len(self._active))
+                self.step()
+            except BaseException as exc:  # reported to waiters
+                with self._cv:
+                    self.loop_error = exc
+                    for req in list(self._queue):
+                        req.error = exc
+                        req._finish("failed", self.clock())
+                    self._queue.clear()
+                    for fld in self._active.values():
+                        if fld.request is not None:
+                            fld.request.error = exc
+                            fld.request._finish("failed", self.clock())
+                    self._active.clear()
+                    self._cv.notify_all()
+                return
+            with self._cv:
+                if not self._active and (len(self._queue), len(self._active)) == before:
+                    self._cv.wait(0.01)  # nothing admissible yet: wait for a change
+
     def release(self, request: Request) -> None:
         """Free a finished kept request's resident field."""
-        fld = request._field
-        if fld is not None and fld.handle in self._idle:
-            del self._idle[fld.handle]
-            self._release_handle(fld.handle)
+        with self._lock:
+            fld = request._field
+            if fld is not None and fld.handle in self._idle:
+                del self._idle[fld.handle]
+                self._release_handle(fld.handle)
 
     def close(self) -> None:
+        if getattr(self, "_thread", None) is not None:
+            self.stop()
         if getattr(self, "_srv", None):
             self._lib.rlic_server_destroy(self._srv)
             self._srv = None
@@ -297,10 +389,9 @@ class Server:
             need = self._field_bytes(dtype, ny, nx, klen, req.keep, passes)
             if need > self.budget_bytes:
                 self._queue.remove(req)
-                req.state = "failed"
                 req.error = MemoryError(f"the field needs {need} bytes, more than the serving "
                                         f"budget of {self.budget_bytes}")
-                req.finished = self.clock()
+                req._finish("failed", self.clock())
                 continue
             if not self._make_room(need):
                 blocked_new = True  # first come, first served for new fields

@@ -165,3 +165,43 @@ def test_device_tensors_and_edge_cases(gpu_device):
     assert np.array_equal(a.result.cpu().numpy(), exp)
     assert np.array_equal(b.result.cpu().numpy(), tex)
     assert c.result.shape == (0, 5)
+
+
+@pytest.mark.gpu
+def test_background_loop_with_concurrent_submitters(gpu_device):
+    import threading
+    import time
+
+    from rlic_amd.serve import Server
+
+    rng = np.random.default_rng(15)
+    reqs = [r for r in _requests(rng, 18) if r["iterations"] > 0]
+    results = {}
+    with Server(1 << 30, device=gpu_device) as srv:
+        srv.start()
+
+        def client(idx):
+            for k in range(idx, len(reqs), 3):
+                h = _submit(srv, reqs[k], keep=(k % 2 == 0))
+                time.sleep(0.002)  # arrivals while earlier requests run
+                assert h.wait(60), k
+                results[k] = h
+                if h.keep:
+                    f = srv.follow_up(h, 2)
+                    assert f.wait(60), k
+                    results[(k, "f")] = f
+
+        threads = [threading.Thread(target=client, args=(i,)) for i in range(3)]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join()
+        srv.run()
+        assert srv.loop_error is None
+        srv.stop()
+    for k, r in enumerate(reqs):
+        assert results[k].state == "done"
+        assert np.array_equal(results[k].result, _expected(r), equal_nan=True), k
+        if (k, "f") in results:
+            exp = _expected(r, r["iterations"] + 2)
+            assert np.array_equal(results[(k, "f")].result, exp, equal_nan=True), k
