@@ -228,6 +228,13 @@ int rlic_autotune_clear(void);
  * counterpart in the reference; for tests. */
 int rlic_div_check_f32(const float* a, const float* b, float* q, int64_t n, void* stream);
 
+/* Scratch memory.  A call's device scratch (packed cells, move record,
+ * ping-pong buffer, queue descriptors) is one block; blocks up to 256 MiB
+ * are cached per (device, stream) and reused by later calls on that stream.
+ * rlic_scratch_release synchronizes the devices concerned and frees the
+ * cached blocks.  No counterpart in the reference. */
+int rlic_scratch_release(void);
+
 /* Library version string.  No counterpart in the reference. */
 const char* rlic_version(void);
 

@@ -96,6 +96,7 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "rlic_autotune_clear": ([], _int),
     "rlic_autotune_load": ([ctypes.c_char_p], _i64),
     "rlic_div_check_f32": ([_vp, _vp, _vp, _i64, _vp], _int),
+    "rlic_scratch_release": ([], _int),
     "rlic_profiler_enable": ([_int], _int),
     "rlic_profiler_read": ([ctypes.c_char_p, _i64], _i64),
     "rlic_version": ([], ctypes.c_char_p),

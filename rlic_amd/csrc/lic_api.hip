@@ -923,6 +923,12 @@ int rlic_div_check_f32(const float* a, const float* b, float* q, int64_t n, void
   return RLIC_OK;
 }
 
+int rlic_scratch_release(void) {
+  t_last_error.clear();
+  RLIC_HIP_TRY(rlic::release_cached_scratch(), "releasing cached scratch");
+  return RLIC_OK;
+}
+
 int64_t rlic_autotune_read(char* buf, int64_t len) {
   std::lock_guard<std::mutex> lock(g_tuner.mu);
   tuner_load_locked();

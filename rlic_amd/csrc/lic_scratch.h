@@ -35,4 +35,8 @@ class CallScratch {
   std::unique_lock<std::mutex> lock_;
 };
 
+// Frees every cached block (after synchronizing the device the blocks
+// belong to); the next calls allocate again.
+hipError_t release_cached_scratch();
+
 }  // namespace rlic

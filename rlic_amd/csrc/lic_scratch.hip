@@ -63,4 +63,25 @@ hipError_t CallScratch::get(size_t bytes, hipStream_t stream) {
   return e;
 }
 
+hipError_t release_cached_scratch() {
+  std::lock_guard<std::mutex> g(cache().mu);
+  hipError_t first = hipSuccess;
+  int current = -1;
+  (void)hipGetDevice(&current);
+  for (auto& kv : cache().blocks) {
+    CachedBlock& b = kv.second;
+    std::lock_guard<std::mutex> busy(b.mu);  // waits for a call that is using it
+    if (!b.ptr) continue;
+    hipError_t e = hipSetDevice(kv.first.first);
+    if (e == hipSuccess) e = hipDeviceSynchronize();  // no call still uses the block
+    if (e == hipSuccess) e = hipFreeAsync(b.ptr, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess && first == hipSuccess) first = e;
+    b.ptr = nullptr;
+    b.bytes = 0;
+  }
+  if (current >= 0) (void)hipSetDevice(current);
+  return first;
+}
+
 }  // namespace rlic

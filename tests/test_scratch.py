@@ -72,3 +72,13 @@ def test_scratch_reuse_sizes_threads_streams(gpu_device):
     for case, exp in cases:
         assert np.array_equal(_run(gpu_device, case, s1), exp)
         assert np.array_equal(_run(gpu_device, case, s2), exp)
+
+
+@pytest.mark.gpu
+def test_scratch_release_then_reuse(gpu_device):
+    from rlic_amd import _native
+
+    case, exp = _case(9, 50, 60, 21, 3)
+    assert np.array_equal(_run(gpu_device, case), exp)
+    _native.raise_for_status(_native.load().rlic_scratch_release())
+    assert np.array_equal(_run(gpu_device, case), exp)  # allocates again
