@@ -154,3 +154,38 @@ def test_batch_device_validation(gpu_device):
         convolve_batch([ok, inf_kernel, neg])
     out = convolve_batch([ok, ok])
     assert all(o.shape == (4, 4) for o in out)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("profile", [0, 1, 2, 3])
+def test_batch_and_serving_follow_each_arithmetic_profile(gpu_device, profile):
+    """Fields with zero and negative velocity components, where the profiles
+    differ (src/lib.rs:157-180), batched and served under each profile, equal
+    the single calls under that profile."""
+    from rlic_amd import _core
+    from rlic_amd.serve import Server
+
+    rng = np.random.default_rng(100 + profile)
+    fields = []
+    for k in range(6):
+        dtype = np.float32 if k % 2 else np.float64
+        ny, nx = 30 + 7 * k, 41 + 3 * k
+        u = np.round(rng.random((ny, nx)) * 4 - 2).astype(dtype)  # many exact zeros and ties
+        v = (rng.random((ny, nx)) - 0.5).astype(dtype)
+        v[:, ::5] = -0.0
+        fields.append(Field(rng.random((ny, nx)).astype(dtype), u, v,
+                            rng.random(int(rng.integers(3, 40))).astype(dtype),
+                            uv_mode=("velocity", "polarization")[k % 2],
+                            boundaries=("closed", "periodic")[k % 3 == 0], iterations=1 + k % 3))
+    outs = convolve_batch(fields, profile=profile)
+    with Server(1 << 28, device=gpu_device, profile=profile) as srv:
+        served = [srv.submit(f.texture, f.u, f.v, kernel=f.kernel, uv_mode=f.uv_mode,
+                             boundaries=f.boundaries, iterations=f.iterations) for f in fields]
+        srv.run()
+    for k, f in enumerate(fields):
+        bs = rlic_amd._boundaries.BoundarySet.from_spec(f.boundaries)
+        fn = _core.convolve_f32 if f.texture.dtype == np.float32 else _core.convolve_f64
+        single = fn(f.texture, (f.u, f.v, f.uv_mode), f.kernel, (bs.x, bs.y), f.iterations,
+                    profile=profile)
+        assert np.array_equal(outs[k], single, equal_nan=True), k
+        assert np.array_equal(served[k].result, single, equal_nan=True), k
