@@ -189,7 +189,10 @@ __global__ __launch_bounds__(256) void lic_replay(const T* __restrict__ xin,
 // (32-bank and 64-bank LDS alike), so an unsheared square reads without
 // bank conflicts.  Moves are decoded two at a time through a 256-entry
 // table indexed by a code byte, holding the two byte-offset deltas as
-// signed 16-bit halves.
+// 32-bit integers (one add per step).  The pair table is the block's only
+// static LDS, at address 0 (a table address is the code byte times 8); the
+// dynamic LDS after it (a multiple of 64 banks further) holds the box, then
+// the box's column map.
 // ---------------------------------------------------------------------------
 constexpr int32_t kLdsThreads = 1024;
 
@@ -200,9 +203,17 @@ __host__ __device__ inline int32_t lds_box_stride(int32_t tw, int32_t klen) {
   return w + ((8 - w) % 32 + 32) % 32;  // == 8 (mod 32)
 }
 
-inline size_t lds_replay_bytes(int32_t th, int32_t tw, int32_t klen, size_t elem) {
+constexpr size_t kLdsPairTableBytes = 256 * sizeof(int2);  // static LDS
+
+// Dynamic LDS of one block: box and column map.
+inline size_t lds_replay_dynamic_bytes(int32_t th, int32_t tw, int32_t klen, size_t elem) {
   return static_cast<size_t>(lds_box_h(th, klen)) * lds_box_stride(tw, klen) * elem +
          static_cast<size_t>(lds_box_w(tw, klen)) * sizeof(int32_t);
+}
+
+// All LDS of one block.
+inline size_t lds_replay_bytes(int32_t th, int32_t tw, int32_t klen, size_t elem) {
+  return kLdsPairTableBytes + lds_replay_dynamic_bytes(th, tw, klen, elem);
 }
 
 __device__ __forceinline__ int32_t wrap_index(int32_t c, int32_t n) {
@@ -222,24 +233,43 @@ __device__ __forceinline__ int32_t lds_move(uint32_t c, const FieldGeom& g, int3
   return d == kMoveXPlus ? elem : d == kMoveXMinus ? -elem : d == kMoveYPlus ? row_bytes : -row_bytes;
 }
 
+// The 8 values one line reads over a group of 8 moves (`word`, one 4-bit
+// move code per step), advancing its pointer `at` into the box.  The
+// pointer (an LDS address) is the line's state, so each step is one add.
+typedef const unsigned char __attribute__((address_space(3))) lds_byte;
+
 template <typename T>
-__device__ __forceinline__ T lds_at(const unsigned char* smem, uint32_t byte_pos) {
-  return *reinterpret_cast<const T*>(smem + byte_pos);
+__device__ __forceinline__ T lds_value(lds_byte* at) {
+  typedef const T __attribute__((address_space(3))) lds_t;
+  return *reinterpret_cast<lds_t*>(at);
+}
+
+template <typename T>
+__device__ __forceinline__ void lds_group(T (&xv)[8], lds_byte*& at, uint32_t word,
+                                          const int2* pair_delta) {
+#pragma unroll
+  for (int h = 0; h < 4; ++h) {
+    const int2 d = pair_delta[(word >> (8 * h)) & 255u];
+    at += d.x;
+    xv[2 * h] = lds_value<T>(at);
+    at += d.y;
+    xv[2 * h + 1] = lds_value<T>(at);
+  }
 }
 
 template <typename T, bool FMA, bool FORWARD, int E>
 __device__ __forceinline__ void replay_directions_lds(
-    T (&acc)[E], const uint32_t (&pos0)[E], const bool (&inside)[E], const unsigned char* box,
-    const T* __restrict__ kernel, int32_t kmid, int32_t nsteps, const uint32_t* pair_delta,
+    T (&acc)[E], lds_byte* const (&at0)[E], const bool (&inside)[E],
+    const T* __restrict__ kernel, int32_t kmid, int32_t nsteps, const int2* pair_delta,
     const uint4* __restrict__ codes, int64_t np, const int64_t (&p)[E], int32_t c0) {
   const int32_t nchunks = chunks_of(nsteps);
   bool alive[E];
-  uint32_t pos[E];
+  lds_byte* at[E];
   uint4 next[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     alive[e] = inside[e];
-    pos[e] = pos0[e];
+    at[e] = at0[e];
     next[e] = (nchunks > 0 && alive[e]) ? codes[c0 * np + p[e]] : make_uint4(0u, 0u, 0u, 0u);
   }
   for (int32_t ch = 0; ch < nchunks; ++ch) {
@@ -261,28 +291,41 @@ __device__ __forceinline__ void replay_directions_lds(
       for (int q = 0; q < 8; ++q)
         kw[q] = kernel[FORWARD ? kmid + 1 + s0 + q : kmid - 1 - s0 - q];
       const int32_t lim = nsteps - s0;
+      uint32_t word[E], stops[E];
+      bool full = lim >= 8;
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        const uint32_t word = alive[e] ? words[e][g] : 0u;  // 0 = stay
-        const uint32_t stops = word & (word >> 1) & (word >> 2) & (word >> 3) & 0x11111111u;
-        const int32_t valid = alive[e] ? (stops ? (__builtin_ctz(stops) >> 2) : 8) : 0;
-        T xv[8];
+        word[e] = alive[e] ? words[e][g] : 0u;  // 0 = stay
+        // nibble 0xF marks the step where the line ended
+        stops[e] = word[e] & (word[e] >> 1) & (word[e] >> 2) & (word[e] >> 3) & 0x11111111u;
+        full = full && alive[e] && stops[e] == 0u;
+      }
+      if (__builtin_amdgcn_ballot_w64(!full) == 0) {
+        // every lane accumulates all 8 steps of every pixel (wave-uniform):
+        // no per-step masks, and every line stays alive
 #pragma unroll
-        for (int h = 0; h < 4; ++h) {
-          const uint32_t d2 = pair_delta[(word >> (8 * h)) & 255u];
-          pos[e] += static_cast<uint32_t>(static_cast<int32_t>(static_cast<int16_t>(d2 & 0xFFFFu)));
-          xv[2 * h] = lds_at<T>(box, pos[e]);
-          pos[e] += static_cast<uint32_t>(static_cast<int32_t>(d2) >> 16);
-          xv[2 * h + 1] = lds_at<T>(box, pos[e]);
-        }
-        // branch-free over the group (see replay_direction)
-        const int32_t use = min(valid, lim);
+        for (int e = 0; e < E; ++e) {
+          T xv[8];
+          lds_group<T>(xv, at[e], word[e], pair_delta);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const T next_acc = macc<FMA>(kw[q], xv[q], acc[e]);
-          acc[e] = q < use ? next_acc : acc[e];
+          for (int q = 0; q < 8; ++q) acc[e] = macc<FMA>(kw[q], xv[q], acc[e]);
         }
-        alive[e] = valid == 8;
+      } else {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int32_t valid =
+              alive[e] ? (stops[e] ? (__builtin_ctz(stops[e]) >> 2) : 8) : 0;
+          T xv[8];
+          lds_group<T>(xv, at[e], word[e], pair_delta);
+          // branch-free over the group (see replay_direction)
+          const int32_t use = min(valid, lim);
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const T next_acc = macc<FMA>(kw[q], xv[q], acc[e]);
+            acc[e] = q < use ? next_acc : acc[e];
+          }
+          alive[e] = valid == 8;
+        }
       }
     }
     bool any = false;
@@ -303,8 +346,9 @@ __global__ __launch_bounds__(kLdsThreads) void lic_replay_lds(const T* __restric
   const int32_t kmid = klen / 2;
   const int32_t bh = lds_box_h(TH, klen), bw = lds_box_w(TW, klen);
   const int32_t stride = lds_box_stride(TW, klen);
-  int32_t* col_map = reinterpret_cast<int32_t*>(smem + sizeof(T) * bh * stride);
-  __shared__ uint32_t pair_delta[256];
+  __shared__ int2 pair_delta[256];
+  unsigned char* box = smem;
+  int32_t* col_map = reinterpret_cast<int32_t*>(box + sizeof(T) * bh * stride);
 
   int32_t ty, tx;
   tile_of_block(blockIdx.x, m, ty, tx);
@@ -314,13 +358,13 @@ __global__ __launch_bounds__(kLdsThreads) void lic_replay_lds(const T* __restric
   if (tid < 256) {
     const int32_t lo = lds_move(tid & 15, g, row_bytes, sizeof(T));
     const int32_t hi = lds_move(tid >> 4, g, row_bytes, sizeof(T));
-    pair_delta[tid] = (static_cast<uint32_t>(hi) << 16) | (static_cast<uint32_t>(lo) & 0xFFFFu);
+    pair_delta[tid] = make_int2(lo, hi);
   }
   for (int32_t b = tid; b < bw; b += kLdsThreads) col_map[b] = wrap_index(j_box0 + b, g.nx);
   __syncthreads();
   // stage the box: wave w copies rows w, w + 16, ... (coalesced along x), four
   // loads in flight per lane
-  T* box_t = reinterpret_cast<T*>(smem);
+  T* box_t = reinterpret_cast<T*>(box);
   const int32_t wave = tid >> 6, lane = tid & 63;
   for (int32_t a = wave; a < bh; a += kLdsThreads / 64) {
     const int64_t src_row = static_cast<int64_t>(wrap_index(i_box0 + a, g.ny)) * g.nx;
@@ -347,7 +391,7 @@ __global__ __launch_bounds__(kLdsThreads) void lic_replay_lds(const T* __restric
   const int32_t n_fwd = klen - 1 - kmid, n_bwd = kmid;
   const int64_t np = static_cast<int64_t>(g.ny) * g.nx;
   T acc[E];
-  uint32_t pos0[E];
+  lds_byte* at0[E];
   bool inside[E];
   int64_t p[E];
 #pragma unroll
@@ -356,12 +400,12 @@ __global__ __launch_bounds__(kLdsThreads) void lic_replay_lds(const T* __restric
     const int32_t i0 = ty * TH + py, j0 = tx * TW + px;
     inside[e] = i0 < g.ny && j0 < g.nx;
     p[e] = inside[e] ? static_cast<int64_t>(i0) * g.nx + j0 : 0;
-    pos0[e] = static_cast<uint32_t>(((kmid + py) * stride + (kmid + px)) * sizeof(T));
-    acc[e] = macc<FMA>(kernel[kmid], lds_at<T>(smem, pos0[e]), T(0));
+    at0[e] = (lds_byte*)box + ((kmid + py) * stride + (kmid + px)) * static_cast<int32_t>(sizeof(T));
+    acc[e] = macc<FMA>(kernel[kmid], lds_value<T>(at0[e]), T(0));
   }
-  replay_directions_lds<T, FMA, true, E>(acc, pos0, inside, smem, kernel, kmid, n_fwd,
+  replay_directions_lds<T, FMA, true, E>(acc, at0, inside, kernel, kmid, n_fwd,
                                          pair_delta, codes, np, p, 0);
-  replay_directions_lds<T, FMA, false, E>(acc, pos0, inside, smem, kernel, kmid, n_bwd,
+  replay_directions_lds<T, FMA, false, E>(acc, at0, inside, kernel, kmid, n_bwd,
                                           pair_delta, codes, np, p, chunks_of(n_fwd));
 #pragma unroll
   for (int e = 0; e < E; ++e)
@@ -538,8 +582,7 @@ LdsTile lds_replay_tile(int32_t klen) {
   const size_t limit = static_cast<size_t>(lds_replay_limit_bytes());
   const int32_t shapes[3][2] = {{64, 64}, {32, 64}, {32, 32}};
   for (const auto& sh : shapes) {
-    // 1 KB of static LDS (pair table) + margin
-    if (lds_replay_bytes(sh[0], sh[1], klen, sizeof(T)) + 2048 <= limit) return {sh[0], sh[1]};
+    if (lds_replay_bytes(sh[0], sh[1], klen, sizeof(T)) <= limit) return {sh[0], sh[1]};
   }
   return {0, 0};
 }
@@ -560,7 +603,7 @@ static void replay_lds_one(const T* x, const T* kernel, int32_t klen, const Fiel
   m.per_xcd = m.n_tiles / kXcds;
   m.rem_xcd = m.n_tiles % kXcds;
   m.band = 4;
-  const size_t bytes = lds_replay_bytes(TH, TW, klen, sizeof(T));
+  const size_t bytes = lds_replay_dynamic_bytes(TH, TW, klen, sizeof(T));
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&lic_replay_lds<T, FMA, TH, TW>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   (void)hipGetLastError();  // the attribute is advisory on this target
@@ -611,7 +654,7 @@ template <typename T>
 hipError_t launch_replay_lds_tile(const T* x, const T* kernel, int32_t klen, const FieldGeom& g,
                                   LdsTile tile, int profile, T* out, const uint4* codes,
                                   hipStream_t stream) {
-  if (lds_replay_bytes(tile.h, tile.w, klen, sizeof(T)) + 2048 >
+  if (lds_replay_bytes(tile.h, tile.w, klen, sizeof(T)) >
       static_cast<size_t>(lds_replay_limit_bytes()))
     return hipErrorInvalidValue;
   const bool ok = (profile & 1)

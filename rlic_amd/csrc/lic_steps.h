@@ -176,16 +176,23 @@ __device__ __forceinline__ T replay_direction(T acc, uint32_t pos,
       T xv[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) xv[q] = x(at[q]);
-      // no break for the last group: a branch here lets the compiler sink
-      // the gathers into it and wait for each one in turn; steps past the
-      // end read "stay" codes (position unchanged, weight from the padding)
-      const int32_t lim = min(valid, nsteps - s0);
+      const bool full = alive && stops == 0u && nsteps - s0 >= 8;
+      if (__builtin_amdgcn_ballot_w64(!full) == 0) {
+        // every lane accumulates all 8 steps (wave-uniform): no masks
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const T next_acc = macc<FMA>(kw[q], xv[q], acc);
-        acc = q < lim ? next_acc : acc;
+        for (int q = 0; q < 8; ++q) acc = macc<FMA>(kw[q], xv[q], acc);
+      } else {
+        // no break for the last group: a break lets the compiler sink the
+        // gathers past it and wait for each one in turn; steps past the end
+        // read "stay" codes (position unchanged, weight from the padding)
+        const int32_t lim = min(valid, nsteps - s0);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const T next_acc = macc<FMA>(kw[q], xv[q], acc);
+          acc = q < lim ? next_acc : acc;
+        }
+        alive = valid == 8;
       }
-      alive = valid == 8;
     }
     if (__builtin_amdgcn_ballot_w64(alive) == 0) break;
   }
