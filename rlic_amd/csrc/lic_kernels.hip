@@ -191,8 +191,7 @@ __global__ __launch_bounds__(256) void lic_replay(const T* __restrict__ xin,
 // table indexed by a code byte, holding the two byte-offset deltas as
 // 32-bit integers (one add per step).  The pair table is the block's only
 // static LDS, at address 0 (a table address is the code byte times 8); the
-// dynamic LDS after it (a multiple of 64 banks further) holds the box, then
-// the box's column map.
+// dynamic LDS after it (a multiple of 64 banks further) holds the box.
 // ---------------------------------------------------------------------------
 constexpr int32_t kLdsThreads = 1024;
 
@@ -205,10 +204,9 @@ __host__ __device__ inline int32_t lds_box_stride(int32_t tw, int32_t klen) {
 
 constexpr size_t kLdsPairTableBytes = 256 * sizeof(int2);  // static LDS
 
-// Dynamic LDS of one block: box and column map.
+// Dynamic LDS of one block: the box.
 inline size_t lds_replay_dynamic_bytes(int32_t th, int32_t tw, int32_t klen, size_t elem) {
-  return static_cast<size_t>(lds_box_h(th, klen)) * lds_box_stride(tw, klen) * elem +
-         static_cast<size_t>(lds_box_w(tw, klen)) * sizeof(int32_t);
+  return static_cast<size_t>(lds_box_h(th, klen)) * lds_box_stride(tw, klen) * elem;
 }
 
 // All LDS of one block.
@@ -302,13 +300,31 @@ __device__ __forceinline__ void replay_directions_lds(
       }
       if (__builtin_amdgcn_ballot_w64(!full) == 0) {
         // every lane accumulates all 8 steps of every pixel (wave-uniform):
-        // no per-step masks, and every line stays alive
+        // no per-step masks, and every line stays alive.  The pair-table
+        // reads of all E lines go first, and line e + 1's gathers are
+        // issued before line e accumulates, so gathers overlap.
+        int2 d[E][4];
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-          T xv[8];
-          lds_group<T>(xv, at[e], word[e], pair_delta);
+        for (int e = 0; e < E; ++e)
 #pragma unroll
-          for (int q = 0; q < 8; ++q) acc[e] = macc<FMA>(kw[q], xv[q], acc[e]);
+          for (int h = 0; h < 4; ++h) d[e][h] = pair_delta[(word[e] >> (8 * h)) & 255u];
+        T xv[2][8];
+#pragma unroll
+        for (int e = 0; e <= E; ++e) {
+          if (e < E) {
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+              at[e] += d[e][h].x;
+              xv[e & 1][2 * h] = lds_value<T>(at[e]);
+              at[e] += d[e][h].y;
+              xv[e & 1][2 * h + 1] = lds_value<T>(at[e]);
+            }
+          }
+          if (e > 0) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+              acc[e - 1] = macc<FMA>(kw[q], xv[(e - 1) & 1][q], acc[e - 1]);
+          }
         }
       } else {
 #pragma unroll
@@ -348,7 +364,6 @@ __global__ __launch_bounds__(kLdsThreads) void lic_replay_lds(const T* __restric
   const int32_t stride = lds_box_stride(TW, klen);
   __shared__ int2 pair_delta[256];
   unsigned char* box = smem;
-  int32_t* col_map = reinterpret_cast<int32_t*>(box + sizeof(T) * bh * stride);
 
   int32_t ty, tx;
   tile_of_block(blockIdx.x, m, ty, tx);
@@ -360,30 +375,38 @@ __global__ __launch_bounds__(kLdsThreads) void lic_replay_lds(const T* __restric
     const int32_t hi = lds_move(tid >> 4, g, row_bytes, sizeof(T));
     pair_delta[tid] = make_int2(lo, hi);
   }
-  for (int32_t b = tid; b < bw; b += kLdsThreads) col_map[b] = wrap_index(j_box0 + b, g.nx);
-  __syncthreads();
-  // stage the box: wave w copies rows w, w + 16, ... (coalesced along x), four
-  // loads in flight per lane
-  T* box_t = reinterpret_cast<T*>(box);
   const int32_t wave = tid >> 6, lane = tid & 63;
-  for (int32_t a = wave; a < bh; a += kLdsThreads / 64) {
-    const int64_t src_row = static_cast<int64_t>(wrap_index(i_box0 + a, g.ny)) * g.nx;
-    for (int32_t b0 = lane; b0 < bw; b0 += 4 * 64) {
-      T v[4];
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int32_t b = b0 + 64 * t;
-        v[t] = b < bw ? x[src_row + col_map[b]] : T(0);
-      }
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int32_t b = b0 + 64 * t;
-        if (b < bw) box_t[a * stride + b] = v[t];
+  // Stage the box with direct global -> LDS loads (no registers, all in
+  // flight at once): wave w copies rows w, w + 16, ... in 256-byte
+  // segments, lane l one dword of each (coalesced along x, wrapped modulo
+  // the image size per element).
+  {
+    constexpr int32_t kDw = static_cast<int32_t>(sizeof(T) / 4);  // dwords per element
+    constexpr int32_t kSegElems = 64 / kDw;                        // elements per segment
+    const int32_t nseg = (bw + kSegElems - 1) / kSegElems;
+    const bool once_x = kmid <= g.nx, once_y = kmid <= g.ny;  // box wraps at most once
+    for (int32_t a = __builtin_amdgcn_readfirstlane(wave); a < bh; a += kLdsThreads / 64) {
+      int32_t r = i_box0 + a;
+      r = once_y ? (r < 0 ? r + g.ny : (r >= g.ny ? r - g.ny : r)) : wrap_index(r, g.ny);
+      const T* src_row = x + static_cast<int64_t>(r) * g.nx;
+      for (int32_t sg = 0; sg < nseg; ++sg) {
+        const int32_t b = sg * kSegElems + lane / kDw;  // element of the box row
+        if (b < bw) {
+          int32_t c = j_box0 + b;
+          c = once_x ? (c < 0 ? c + g.nx : (c >= g.nx ? c - g.nx : c)) : wrap_index(c, g.nx);
+          const unsigned char* src =
+              reinterpret_cast<const unsigned char*>(src_row + c) + 4 * (lane % kDw);
+          __builtin_amdgcn_global_load_lds(
+              (const unsigned __attribute__((address_space(1)))*)src,
+              (unsigned __attribute__((address_space(3)))*)(box + (a * stride + sg * kSegElems) *
+                                                                       static_cast<int32_t>(sizeof(T))),
+              4, 0, 0);
+        }
       }
     }
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's copies have landed
   }
   __syncthreads();
-
   // wave w covers the 8 x 8 square at (8 * (w / 4), 8 * (w % 4)) of each
   // 32 x 32 quarter e of the tile
   const int32_t py0 = ((wave >> 2) << 3) | (lane >> 3);
