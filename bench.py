@@ -55,6 +55,28 @@ def traffic_from_profiles(kernel: str, workload: str) -> float | None:
         return None
 
 
+def measure_copy_gbs(dev, repeats: int = 10) -> float:
+    """Measured HBM bandwidth of a plain device copy (STREAM copy): PyTorch's
+    copy of 2^28 float32 (1 GiB read, 1 GiB written), timed with events on
+    the stream it runs on."""
+    import torch
+
+    src = torch.rand((1 << 28,), device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    stream = torch.cuda.current_stream(dev)
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(repeats):
+        dst.copy_(src)
+    end.record(stream)
+    end.synchronize()
+    avg_s = start.elapsed_time(end) / repeats / 1e3
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2 * (1 << 30) / avg_s / 1e9
+
+
 def cpu_baseline(wl, rows: int) -> dict:
     """Time the CPU oracle on pass 1 of a band of `rows` rows (one thread)."""
     from oracle import cpu
@@ -361,6 +383,11 @@ def main() -> int:
             "kernel_ms_per_step": {k: round(v["total_ms"] / args.steps, 4)
                                    for k, v in prof.items()},
         }
+
+    if roofline is not None:
+        # untimed: what a plain copy reaches on this HBM, beside the nominal peak
+        roofline["measured_copy_gbs"] = round(measure_copy_gbs(dev), 1)
+        log(f"bench: plain copy {roofline['measured_copy_gbs']} GB/s")
 
     # divergence (untimed): lane work of one tracing pass, read back from its
     # move record (RLIC_TRACE_STATS=1)

@@ -136,6 +136,23 @@ __device__ __forceinline__ uint32_t lane_copy(uint32_t x) {
   return r;
 }
 
+// 2 * x + (this lane's bit of the wave64 lane mask `mask`) in one VALU
+// instruction: the mask is the add's carry-in.
+__device__ __forceinline__ uint32_t carry_in_bit(uint32_t x, uint64_t mask) {
+  uint32_t r;
+  uint64_t carry_out;
+  asm("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(r), "=s"(carry_out) : "v"(x), "s"(mask));
+  return r;
+}
+
+// This lane's bit of the wave64 lane mask `mask`, as 0 or 1.
+__device__ __forceinline__ uint32_t lane_bit(uint64_t mask) {
+  uint32_t r;
+  uint64_t carry_out;
+  asm("v_addc_co_u32_e64 %0, %1, 0, 0, %2" : "=v"(r), "=s"(carry_out) : "s"(mask));
+  return r;
+}
+
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float abs_t(float a) { return __builtin_fabsf(a); }
@@ -201,19 +218,29 @@ struct StepGeom {
   bool x_lo_per, x_hi_per, y_lo_per, y_hi_per;  // periodic sides
 };
 
-// Move codes recorded by the tracing pass (4 bits per step, 8 per word):
-// the direction, plus kMoveEdge when the step left the image (the boundary
-// rule of that side then applies), kMoveStay for no move and kMoveStop where
-// the streamline ended on a NaN vector (src/lib.rs:336-338).
+// Move codes recorded by the tracing pass (4 bits per step, 8 per word), a
+// bit field: kMoveMoved for a step to a neighbouring cell, plus kMoveAxisY
+// for a step along y (rows), kMoveNeg toward lower indices and kMoveEdge
+// when the step left the image (the boundary rule of that side then
+// applies); kMoveStay for no move and kMoveStop where the streamline ended
+// on a NaN vector (src/lib.rs:336-338).  Codes 1-6 do not occur.
 enum : uint32_t {
   kMoveStay = 0,
-  kMoveXPlus = 1,
-  kMoveXMinus = 2,
-  kMoveYPlus = 3,
-  kMoveYMinus = 4,
-  kMoveEdge = 8,
-  kMoveStop = 15
+  kMoveNeg = 1,
+  kMoveAxisY = 2,
+  kMoveEdge = 4,
+  kMoveStop = 7,
+  kMoveMoved = 8,
+  kMoveXPlus = kMoveMoved,
+  kMoveXMinus = kMoveMoved | kMoveNeg,
+  kMoveYPlus = kMoveMoved | kMoveAxisY,
+  kMoveYMinus = kMoveMoved | kMoveAxisY | kMoveNeg
 };
+
+// Bit 4k set where nibble k of `word` is the stop code.
+__host__ __device__ inline uint32_t stop_nibbles(uint32_t word) {
+  return ~(word >> 3) & (word >> 2) & (word >> 1) & word & 0x11111111u;
+}
 
 // Positions are in `unit`s: 1 for element indices, the element size for
 // byte offsets.

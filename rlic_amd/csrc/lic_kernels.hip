@@ -223,12 +223,12 @@ __device__ __forceinline__ int32_t wrap_index(int32_t c, int32_t n) {
 // unwrapped coordinates a periodic edge step is an ordinary step.
 __device__ __forceinline__ int32_t lds_move(uint32_t c, const FieldGeom& g, int32_t row_bytes,
                                             int32_t elem) {
-  const bool edge = (c & kMoveEdge) != 0 && c != kMoveStop;
-  const uint32_t d = c & 7u;
-  const bool per = d == kMoveXPlus ? g.x_right : d == kMoveXMinus ? g.x_left
-                 : d == kMoveYPlus ? g.y_right : g.y_left;
-  if (c == kMoveStop || d == 0 || d > 4 || (edge && !per)) return 0;
-  return d == kMoveXPlus ? elem : d == kMoveXMinus ? -elem : d == kMoveYPlus ? row_bytes : -row_bytes;
+  if ((c & kMoveMoved) == 0) return 0;  // stay, stop (and the unused codes)
+  const bool y = (c & kMoveAxisY) != 0, neg = (c & kMoveNeg) != 0;
+  const bool per = y ? (neg ? g.y_left : g.y_right) : (neg ? g.x_left : g.x_right);
+  if ((c & kMoveEdge) != 0 && !per) return 0;  // a closed edge keeps the cell
+  const int32_t d = y ? row_bytes : elem;
+  return neg ? -d : d;
 }
 
 // The 8 values one line reads over a group of 8 moves (`word`, one 4-bit
@@ -295,7 +295,7 @@ __device__ __forceinline__ void replay_directions_lds(
       for (int e = 0; e < E; ++e) {
         word[e] = alive[e] ? words[e][g] : 0u;  // 0 = stay
         // nibble 0xF marks the step where the line ended
-        stops[e] = word[e] & (word[e] >> 1) & (word[e] >> 2) & (word[e] >> 3) & 0x11111111u;
+        stops[e] = stop_nibbles(word[e]);
         full = full && alive[e] && stops[e] == 0u;
       }
       if (__builtin_amdgcn_ballot_w64(!full) == 0) {

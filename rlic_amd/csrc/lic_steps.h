@@ -115,13 +115,26 @@ __device__ __forceinline__ T trace_direction(T acc, T pu, T pv, uint32_t row, ui
         pu = cv.u;
         pv = cv.v;
         if (REC != 0 && rec) {
-          const uint32_t code_x = (pos_x ? kMoveXPlus : kMoveXMinus) | (edge_x ? kMoveEdge : 0u);
-          const uint32_t code_y = (pos_y ? kMoveYPlus : kMoveYMinus) | (edge_y ? kMoveEdge : 0u);
-          uint32_t code = moves ? (xs ? code_x : code_y) : kMoveStay;
-          code = (alive && ends) ? kMoveStop : code;
-          // the copy keeps the selects on 4-bit constants (inline operands)
-          // instead of 8 shifted copies of each, which would occupy VGPRs
-          word |= lane_copy(code) << (4 * q);
+          // The code's four bits (moved, edge, y axis, negative direction;
+          // the three low bits alone for the stop code) are combined from
+          // the step's comparisons as wave lane masks on the scalar unit,
+          // then shifted in one at a time with an add-with-carry (code =
+          // 2 * code + bit).  A line that has ended stays on its NaN cell,
+          // so it records the stop code again at every later step; readers
+          // stop at the first one.
+          const uint64_t m_stop = __builtin_amdgcn_ballot_w64(ends);
+          const uint64_t m_nz = __builtin_amdgcn_ballot_w64(mu != T(0)) |
+                                __builtin_amdgcn_ballot_w64(mv != T(0));
+          const uint64_t m_x = __builtin_amdgcn_ballot_w64(xs);
+          const uint64_t m_edge = (m_x & __builtin_amdgcn_ballot_w64(edge_x)) |
+                                  (~m_x & __builtin_amdgcn_ballot_w64(edge_y));
+          const uint64_t m_neg = ~((m_x & __builtin_amdgcn_ballot_w64(pos_x)) |
+                                   (~m_x & __builtin_amdgcn_ballot_w64(pos_y)));
+          uint32_t code = lane_bit(m_nz & ~m_stop);
+          code = carry_in_bit(code, (m_nz & m_edge) | m_stop);
+          code = carry_in_bit(code, (m_nz & ~m_x) | m_stop);
+          code = carry_in_bit(code, (m_nz & m_neg) | m_stop);
+          word |= code << (4 * q);
         }
         alive = live;
       }
@@ -161,7 +174,7 @@ __device__ __forceinline__ T replay_direction(T acc, uint32_t pos,
       if (s0 >= nsteps) break;  // uniform
       const uint32_t word = alive ? words[g] : 0u;  // 0 = stay
       // nibble 0xF marks the step where the line ended
-      const uint32_t stops = word & (word >> 1) & (word >> 2) & (word >> 3) & 0x11111111u;
+      const uint32_t stops = stop_nibbles(word);
       const int32_t valid = alive ? (stops ? (__builtin_ctz(stops) >> 2) : 8) : 0;
       T kw[8];
 #pragma unroll
