@@ -242,6 +242,14 @@ __host__ __device__ inline uint32_t stop_nibbles(uint32_t word) {
   return ~(word >> 3) & (word >> 2) & (word >> 1) & word & 0x11111111u;
 }
 
+// Whether a word of 8 recorded steps holds a stop code.  The tracing pass
+// records the stop code again at every step after a line has ended (the
+// line stays on its NaN cell), so a word whose 8 steps were all recorded
+// has one exactly when its last code is the stop code.
+__host__ __device__ inline bool group_has_stop(uint32_t word) {
+  return (word >> 28) == kMoveStop;
+}
+
 // Positions are in `unit`s: 1 for element indices, the element size for
 // byte offsets.
 __host__ __device__ inline StepGeom make_step_geom(const FieldGeom& g, uint32_t unit) {

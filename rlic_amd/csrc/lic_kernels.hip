@@ -289,14 +289,12 @@ __device__ __forceinline__ void replay_directions_lds(
       for (int q = 0; q < 8; ++q)
         kw[q] = kernel[FORWARD ? kmid + 1 + s0 + q : kmid - 1 - s0 - q];
       const int32_t lim = nsteps - s0;
-      uint32_t word[E], stops[E];
+      uint32_t word[E];
       bool full = lim >= 8;
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         word[e] = alive[e] ? words[e][g] : 0u;  // 0 = stay
-        // nibble 0xF marks the step where the line ended
-        stops[e] = stop_nibbles(word[e]);
-        full = full && alive[e] && stops[e] == 0u;
+        full = full && alive[e] && !group_has_stop(word[e]);
       }
       if (__builtin_amdgcn_ballot_w64(!full) == 0) {
         // every lane accumulates all 8 steps of every pixel (wave-uniform):
@@ -329,8 +327,8 @@ __device__ __forceinline__ void replay_directions_lds(
       } else {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-          const int32_t valid =
-              alive[e] ? (stops[e] ? (__builtin_ctz(stops[e]) >> 2) : 8) : 0;
+          const uint32_t stops = stop_nibbles(word[e]);
+          const int32_t valid = alive[e] ? (stops ? (__builtin_ctz(stops) >> 2) : 8) : 0;
           T xv[8];
           lds_group<T>(xv, at[e], word[e], pair_delta);
           // branch-free over the group (see replay_direction)

@@ -173,9 +173,6 @@ __device__ __forceinline__ T replay_direction(T acc, uint32_t pos,
       const int32_t s0 = ch * kChunkSteps + 8 * g;
       if (s0 >= nsteps) break;  // uniform
       const uint32_t word = alive ? words[g] : 0u;  // 0 = stay
-      // nibble 0xF marks the step where the line ended
-      const uint32_t stops = stop_nibbles(word);
-      const int32_t valid = alive ? (stops ? (__builtin_ctz(stops) >> 2) : 8) : 0;
       T kw[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q)
@@ -189,7 +186,7 @@ __device__ __forceinline__ T replay_direction(T acc, uint32_t pos,
       T xv[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) xv[q] = x(at[q]);
-      const bool full = alive && stops == 0u && nsteps - s0 >= 8;
+      const bool full = alive && !group_has_stop(word) && nsteps - s0 >= 8;
       if (__builtin_amdgcn_ballot_w64(!full) == 0) {
         // every lane accumulates all 8 steps (wave-uniform): no masks
 #pragma unroll
@@ -198,6 +195,8 @@ __device__ __forceinline__ T replay_direction(T acc, uint32_t pos,
         // no break for the last group: a break lets the compiler sink the
         // gathers past it and wait for each one in turn; steps past the end
         // read "stay" codes (position unchanged, weight from the padding)
+        const uint32_t stops = stop_nibbles(word);
+        const int32_t valid = alive ? (stops ? (__builtin_ctz(stops) >> 2) : 8) : 0;
         const int32_t lim = min(valid, nsteps - s0);
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
