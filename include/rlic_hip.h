@@ -123,9 +123,11 @@ typedef struct {
 } rlic_field_f64;
 
 /* Batched convolution: n_fields independent rlic_convolve_* calls (device
- * memory only) run together in one persistent launch fed by a device-side
- * work queue.  Results are the same, bit for bit, as the single calls.
- * Returns after the work has completed on `stream`.  No counterpart in the
+ * memory only).  Fields of at least 2^18 pixels (RLIC_BATCH_SINGLE_MIN)
+ * run as single-field launches; the rest run together in one persistent
+ * launch fed by a device-side work queue.  Results are the same, bit for
+ * bit, as the single calls.  Returns after the work has completed on
+ * `stream`.  No counterpart in the
  * reference (which has one call per field, src/lib.rs:450-482); each field
  * follows the semantics of that call. */
 int rlic_convolve_batch_f32(const rlic_field_f32* fields, int64_t n_fields, int profile,

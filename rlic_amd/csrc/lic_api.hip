@@ -603,12 +603,44 @@ int batch_entry(const FieldSpec* fields, int64_t nfields, int profile, void* str
   RLIC_HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
   configure_pool(dev);
   hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+  // Fields of at least RLIC_BATCH_SINGLE_MIN pixels (default 2^18; 0 or
+  // less: none) fill the GPU on their own: they run as single-field launches
+  // (trace, then LDS-staged or tuned replays), ahead of one persistent
+  // launch shared by the rest.  Either way a field gives the bits of its
+  // single call.
+  const int single_min = env_int("RLIC_BATCH_SINGLE_MIN", 1 << 18);
+  std::vector<FieldSpec> rest;
+  rest.reserve(static_cast<size_t>(nfields));
   rlic::BatchScratch scratch;
   std::string err;
-  int rc;
+  int rc = RLIC_OK;
   {
     LaunchTimer timer("lic_batch", stream);
-    rc = rlic::batch_run<T, FieldSpec>(fields, nfields, profile, stream, scratch, err);
+    for (int64_t k = 0; k < nfields && rc == RLIC_OK; ++k) {
+      const FieldSpec& f = fields[k];
+      if (single_min <= 0 || f.ny * f.nx < single_min) {
+        rest.push_back(f);
+        continue;
+      }
+      const rlic::FieldGeom g{static_cast<int32_t>(f.ny), static_cast<int32_t>(f.nx),
+                              f.x_left, f.x_right, f.y_left, f.y_right};
+      rc = run_on_device<T>(f.texture, f.tex_sy, f.tex_sx, f.u, f.u_sy, f.u_sx, f.v, f.v_sy,
+                            f.v_sx, static_cast<int32_t>(f.ny), static_cast<int32_t>(f.nx),
+                            f.kernel, static_cast<int32_t>(f.kernel_len), f.uv_mode, g,
+                            f.iterations, f.out, profile, stream);
+      if (rc != RLIC_OK) {
+        t_last_error = "field " + std::to_string(k) + ": " + t_last_error;
+        return rc;
+      }
+    }
+    if (!rest.empty()) {
+      rc = rlic::batch_run<T, FieldSpec>(rest.data(), static_cast<int64_t>(rest.size()), profile,
+                                         stream, scratch, err);
+    } else if (hipError_t e = hipStreamSynchronize(stream); e != hipSuccess) {
+      (void)hipGetLastError();
+      rc = RLIC_ERR_HIP;
+      err = std::string("batched call: ") + hipGetErrorString(e);
+    }
   }
   if (rc != RLIC_OK) {
     (void)hipStreamSynchronize(stream);

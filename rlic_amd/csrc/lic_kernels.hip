@@ -382,7 +382,11 @@ __global__ __launch_bounds__(kLdsThreads) void lic_replay_lds(const T* __restric
     constexpr int32_t kDw = static_cast<int32_t>(sizeof(T) / 4);  // dwords per element
     constexpr int32_t kSegElems = 64 / kDw;                        // elements per segment
     const int32_t nseg = (bw + kSegElems - 1) / kSegElems;
-    const bool once_x = kmid <= g.nx, once_y = kmid <= g.ny;  // box wraps at most once
+    // one add or subtract wraps a box coordinate when the whole box lies
+    // within one image size of the image (rows [-ny, 2 ny), columns alike);
+    // otherwise (images smaller than the box) the general modulo
+    const bool once_y = i_box0 >= -g.ny && int64_t{i_box0} + bh <= 2 * int64_t{g.ny};
+    const bool once_x = j_box0 >= -g.nx && int64_t{j_box0} + bw <= 2 * int64_t{g.nx};
     for (int32_t a = __builtin_amdgcn_readfirstlane(wave); a < bh; a += kLdsThreads / 64) {
       int32_t r = i_box0 + a;
       r = once_y ? (r < 0 ? r + g.ny : (r >= g.ny ? r - g.ny : r)) : wrap_index(r, g.ny);

@@ -49,6 +49,26 @@ def test_batch_matches_single_calls():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("single_min", ["1", "3000"])
+def test_batch_single_field_launches_match_single_calls(monkeypatch, single_min):
+    """Fields at or above RLIC_BATCH_SINGLE_MIN pixels run as single-field
+    launches ahead of the persistent launch of the rest: every field, and
+    the mix, gives the single calls' bits."""
+    monkeypatch.setenv("RLIC_BATCH_SINGLE_MIN", single_min)
+    rng = np.random.default_rng(5)
+    fields = _mixed_fields(rng, 24)
+    sizes = [f.texture.size for f in fields]
+    assert any(s >= int(single_min) for s in sizes)
+    if single_min != "1":
+        assert any(s < int(single_min) for s in sizes)
+    outs = convolve_batch(fields)
+    for k, (f, out) in enumerate(zip(fields, outs)):
+        single = rlic_amd.convolve(f.texture, f.u, f.v, kernel=f.kernel, uv_mode=f.uv_mode,
+                                   boundaries=f.boundaries, iterations=f.iterations)
+        assert np.array_equal(out, single, equal_nan=True), f"field {k} differs"
+
+
+@pytest.mark.gpu
 def test_batch_matches_oracle_on_device_tensors(gpu_device):
     import torch
 
