@@ -74,8 +74,8 @@ __device__ __forceinline__ T trace_direction(T acc, T pu, T pv, uint32_t row, ui
       for (int q = 0; q < 8; ++q) {
         if (s0 + q >= nsteps) break;  // uniform: last group only
         // a NaN component ends the line (src/lib.rs:336-338)
-        const bool ends = __builtin_isunordered(pu, pv);
-        const bool live = alive && !ends;
+        const bool ordered = !__builtin_isunordered(pu, pv);
+        const bool live = alive && ordered;
         if (POL != 0 && pol) {  // src/lib.rs:339-347, unfused dot product
           const T d1 = pu * last_u;
           const T d2 = pv * last_v;
@@ -94,7 +94,12 @@ __device__ __forceinline__ T trace_direction(T acc, T pu, T pv, uint32_t row, ui
         // no move when both components are +-0 (src/lib.rs:242-244)
         const bool moves = live && (mu != T(0) || mv != T(0));
         // update_state (src/lib.rs:208-233) for either axis, then keep one
-        const bool pos_x = mu >= T(0);
+        // pos_x only matters when the x edge comes first (tx < ty), which
+        // needs a finite tx and so a nonzero, non-NaN mu: there it equals
+        // mu >= 0 (src/lib.rs:217), and the sign test is the one the step
+        // time already made.  pos_y is used for y steps, which also happen
+        // with mv = -0 (tx and ty both infinite), so it stays mv >= 0.
+        const bool pos_x = !__builtin_signbit(mu);
         const bool pos_y = mv >= T(0);
         const T fy_after_x = macc<FMA>(tx, mv, fy);
         const T fx_after_y = macc<FMA>(ty, mu, fx);
@@ -102,11 +107,15 @@ __device__ __forceinline__ T trace_direction(T acc, T pu, T pv, uint32_t row, ui
         const T fy_new = xs ? fy_after_x : (pos_y ? T(0) : T(1));
         fx = moves ? fx_new : fx;
         fy = moves ? fy_new : fy;
-        // the crossed coordinate with the boundary rule (src/lib.rs:83-95)
-        const bool edge_x = j == (pos_x ? nxm1 : 0u);
-        const uint32_t j_to = edge_x ? (pos_x ? x_hi : x_lo) : j + (pos_x ? unit : 0u - unit);
-        const bool edge_y = row == (pos_y ? last_row : 0u);
-        const uint32_t row_to = edge_y ? (pos_y ? y_hi : y_lo) : row + (pos_y ? row_up : row_down);
+        // the crossed coordinate with the boundary rule (src/lib.rs:83-95):
+        // a step off an edge leaves [0, last] (unsigned, so stepping below 0
+        // wraps to a huge value), and the side's rule picks the target
+        const uint32_t j_step = j + (pos_x ? unit : 0u - unit);
+        const bool edge_x = j_step > nxm1;
+        const uint32_t j_to = edge_x ? (pos_x ? x_hi : x_lo) : j_step;
+        const uint32_t row_step = row + (pos_y ? row_up : row_down);
+        const bool edge_y = row_step > last_row;
+        const uint32_t row_to = edge_y ? (pos_y ? y_hi : y_lo) : row_step;
         j = (moves && xs) ? j_to : j;
         row = (moves && !xs) ? row_to : row;
         const CellVal<T> cv = cell(row + j);
@@ -122,7 +131,7 @@ __device__ __forceinline__ T trace_direction(T acc, T pu, T pv, uint32_t row, ui
           // 2 * code + bit).  A line that has ended stays on its NaN cell,
           // so it records the stop code again at every later step; readers
           // stop at the first one.
-          const uint64_t m_stop = __builtin_amdgcn_ballot_w64(ends);
+          const uint64_t m_stop = ~__builtin_amdgcn_ballot_w64(ordered);
           const uint64_t m_nz = __builtin_amdgcn_ballot_w64(mu != T(0)) |
                                 __builtin_amdgcn_ballot_w64(mv != T(0));
           const uint64_t m_x = __builtin_amdgcn_ballot_w64(xs);

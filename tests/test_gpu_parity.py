@@ -305,3 +305,20 @@ def test_replay_tile_shapes(dtype, klen):
         got = rlic_amd.convolve(tex, u, v, kernel=k, uv_mode="polarization", boundaries=b,
                                 iterations=2)
         _assert_same(got, exp, f"{np.dtype(dtype).name} L={klen} {b}")
+
+
+@pytest.mark.parametrize("shape", [(33, 1), (14, 107), (65, 40), (63, 129), (130, 3), (128, 64)])
+def test_lds_box_wraps_around_small_images(shape):
+    """The LDS replay stages a box of tile + kmid halo wrapped modulo the
+    image: with one add or subtract when the box lies within one image size
+    of the image, else with a modulo.  Images around and below the box size,
+    periodic on both axes so that lines cross the edges, both dtypes."""
+    rng = np.random.default_rng(shape[0] * 1000 + shape[1])
+    for dtype in (np.float32, np.float64):
+        tex, u, v = _fields(rng, shape, dtype, "holes")
+        for klen in (3, 17, 33, 65):
+            k = rng.random(klen).astype(dtype)
+            for b in ("periodic", {"x": "periodic", "y": "closed"}):
+                exp = cpu.convolve(tex, u, v, kernel=k, boundaries=b, iterations=3)
+                got = rlic_amd.convolve(tex, u, v, kernel=k, boundaries=b, iterations=3)
+                _assert_same(got, exp, f"{shape} {np.dtype(dtype).name} L={klen} {b}")
