@@ -188,18 +188,26 @@ __device__ __forceinline__ double div_rn(double a, double b) { return a / b; }
 
 // `velocity` is never NaN here: NaN vectors end the streamline first
 // (src/lib.rs:336-338).
+// `negative` is the sign bit of `velocity` (the branchless rule's
+// 1 + signum(velocity)), given by the caller, which tests it once for
+// both uses.
 template <typename T, bool FMA, bool BRANCHLESS>
-__device__ __forceinline__ T time_to_next_pixel(T velocity, T frac) {
+__device__ __forceinline__ T time_to_next_pixel(T velocity, T frac, bool negative) {
   if constexpr (!BRANCHLESS) {
     if (velocity > T(0)) return div_rn(T(1) - frac, velocity);
     if (velocity < T(0)) return -div_rn(frac, velocity);
     return inf_t<T>();
   } else {
     // 1 + signum(velocity): 0 when the sign bit is set (also -0.0), else 2.
-    const T factor = __builtin_signbit(velocity) ? T(0) : T(2);
+    const T factor = negative ? T(0) : T(2);
     const T remaining = macc<FMA>(factor, T(0.5) - frac, frac);
     return abs_t(div_rn(remaining, velocity));
   }
+}
+
+template <typename T, bool FMA, bool BRANCHLESS>
+__device__ __forceinline__ T time_to_next_pixel(T velocity, T frac) {
+  return time_to_next_pixel<T, FMA, BRANCHLESS>(velocity, frac, __builtin_signbit(velocity));
 }
 
 struct FieldGeom {

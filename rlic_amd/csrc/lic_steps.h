@@ -87,7 +87,8 @@ __device__ __forceinline__ T trace_direction(T acc, T pu, T pv, uint32_t row, ui
         }
         const T mu = FORWARD ? pu : -pu;  // src/lib.rs:348-351
         const T mv = FORWARD ? pv : -pv;
-        const T tx = time_to_next_pixel<T, FMA, BRANCHLESS>(mu, fx);
+        const bool neg_x = __builtin_signbit(mu);
+        const T tx = time_to_next_pixel<T, FMA, BRANCHLESS>(mu, fx, neg_x);
         const T ty = time_to_next_pixel<T, FMA, BRANCHLESS>(mv, fy);
         // x edge first when tx < ty; ties and NaN go to y (src/lib.rs:249-269)
         const bool xs = tx < ty;
@@ -99,7 +100,7 @@ __device__ __forceinline__ T trace_direction(T acc, T pu, T pv, uint32_t row, ui
         // mu >= 0 (src/lib.rs:217), and the sign test is the one the step
         // time already made.  pos_y is used for y steps, which also happen
         // with mv = -0 (tx and ty both infinite), so it stays mv >= 0.
-        const bool pos_x = !__builtin_signbit(mu);
+        const bool pos_x = !neg_x;
         const bool pos_y = mv >= T(0);
         const T fy_after_x = macc<FMA>(tx, mv, fy);
         const T fx_after_y = macc<FMA>(ty, mu, fx);
