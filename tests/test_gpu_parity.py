@@ -322,3 +322,29 @@ def test_lds_box_wraps_around_small_images(shape):
                 exp = cpu.convolve(tex, u, v, kernel=k, boundaries=b, iterations=3)
                 got = rlic_amd.convolve(tex, u, v, kernel=k, boundaries=b, iterations=3)
                 _assert_same(got, exp, f"{shape} {np.dtype(dtype).name} L={klen} {b}")
+
+
+@pytest.mark.parametrize("profile", [2, 3])
+def test_extreme_velocity_magnitudes(profile):
+    """Float, branchless profiles: velocities of extreme magnitude (2^-75 to
+    2^75), exact zeros, denormals, values near the float maximum and NaN
+    holes scattered over a field, so that step times over- and underflow
+    and fractions leave [0, 1]; every mode and boundary set must give the
+    oracle's bits."""
+    rng = np.random.default_rng(60 + profile)
+    shape = (96, 160)
+    tex, u, v = _fields(rng, shape, np.float32, "holes")
+    scale = np.exp2(rng.uniform(-75, 75, shape))
+    pick = rng.random(shape)
+    u = np.where(pick < 0.08, u * scale, u).astype(np.float32)
+    v = np.where((pick > 0.92), v * scale, v).astype(np.float32)
+    u[rng.random(shape) < 0.01] = np.float32(1e-42)   # denormal
+    v[rng.random(shape) < 0.01] = np.float32(3e38)
+    for klen in (9, 33, 65):
+        k = rng.random(klen).astype(np.float32)
+        for uv_mode in MODES:
+            for b in BOUNDS:
+                exp = cpu.convolve(tex, u, v, kernel=k, uv_mode=uv_mode, boundaries=b,
+                                   iterations=2, profile=profile)
+                got = _core.convolve_f32(tex, (u, v, uv_mode), k, _bpair(b), 2, profile=profile)
+                _assert_same(got, exp, f"profile={profile} L={klen} {uv_mode} {b}")
