@@ -1,10 +1,12 @@
 """Many convolutions in one persistent GPU launch.
 
 ``convolve_batch(fields)`` runs one ``convolve`` per field — each field with
-its own size, kernel, boundaries, uv_mode and iteration count — in a single
-persistent launch fed by a device-side work queue (``rlic_convolve_batch_*``
-of ``librlic_hip.so``), instead of one launch sequence per field.  Each result
-is bit-identical to the corresponding ``rlic_amd.convolve`` call.
+its own size, kernel, boundaries, uv_mode and iteration count — through
+``rlic_convolve_batch_*`` of ``librlic_hip.so``: fields of at least 2^18
+pixels run as single-field launches, all others together in one persistent
+launch fed by a device-side work queue, instead of one launch sequence per
+field.  Each result is bit-identical to the corresponding
+``rlic_amd.convolve`` call.
 
 Fields may hold NumPy arrays (uploaded to ``device`` and returned as NumPy)
 or ROCm PyTorch tensors (returned as tensors on their device).  Every field
@@ -23,7 +25,8 @@ from typing import Any, Sequence
 import numpy as np
 
 from rlic_amd import _core, _native
-from rlic_amd._lib import validate
+from rlic_amd._boundaries import BoundarySet
+from rlic_amd._lib import _KNOWN_UV_MODES, validate
 
 _MODE = {"velocity": 0, "polarization": 1}
 _BOUND = {"closed": 0, "periodic": 1}
@@ -50,6 +53,55 @@ def _as_field(f) -> Field:
         return Field(**f)
     texture, u, v = f[:3]
     return Field(texture, u, v, **(f[3] if len(f) > 3 else {}))
+
+
+_TORCH_FLOAT_DTYPES: dict = {}
+_BOUNDARY_CACHE: dict = {}
+
+
+def _spec_key(spec):
+    """A hashable form of a boundary specification, or None."""
+    if isinstance(spec, str):
+        return spec
+    if isinstance(spec, dict):
+        try:
+            return tuple(sorted((k, tuple(v) if isinstance(v, (list, tuple)) else v)
+                                for k, v in spec.items()))
+        except TypeError:
+            return None
+    return None
+
+
+def _quick_structure_check(f: Field):
+    """``validate(..., check_values=False)``'s result for a field of ROCm
+    tensors that passes every structural check (float32/float64 throughout,
+    2-D texture, u, v of its shape, 1-D kernel, iterations >= 0, a known
+    uv_mode, a valid boundary specification), or None to run ``validate``."""
+    if not _TORCH_FLOAT_DTYPES:
+        import torch
+
+        _TORCH_FLOAT_DTYPES.update({torch.float32: np.dtype("float32"),
+                                    torch.float64: np.dtype("float64")})
+    t, u, v, k = f.texture, f.u, f.v, f.kernel
+    if not all(_core._is_torch(a) for a in (t, u, v, k)):
+        return None
+    dtype = _TORCH_FLOAT_DTYPES.get(t.dtype)
+    if dtype is None or u.dtype != t.dtype or v.dtype != t.dtype or k.dtype != t.dtype:
+        return None
+    if t.dim() != 2 or k.dim() != 1 or u.shape != t.shape or v.shape != t.shape:
+        return None
+    if not isinstance(f.iterations, int) or f.iterations < 0 or f.uv_mode not in _KNOWN_UV_MODES:
+        return None
+    key = _spec_key(f.boundaries)
+    if key is None:
+        return None
+    bs = _BOUNDARY_CACHE.get(key)
+    if bs is None:
+        bs = BoundarySet.from_spec(f.boundaries)
+        if bs is None or bs.collect_exceptions():
+            return None
+        _BOUNDARY_CACHE[key] = bs
+    return bs, dtype
 
 
 def _first_value_problem(items: list[Field], host: bool) -> int | None:
@@ -101,6 +153,10 @@ def convolve_batch(fields: Sequence, *, profile: int | str | None = None, device
         arrays = (f.texture, f.u, f.v, f.kernel)
         if any(_core._is_torch(a) for a in arrays) == host:
             raise TypeError("a batch holds NumPy arrays or torch tensors, not both")
+        quick = None if host else _quick_structure_check(f)
+        if quick is not None:
+            checked.append(quick)
+            continue
         try:
             checked.append(validate(f.texture, f.u, f.v, kernel=f.kernel, uv_mode=f.uv_mode,
                                     boundaries=f.boundaries, iterations=f.iterations,

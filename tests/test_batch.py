@@ -209,3 +209,44 @@ def test_batch_and_serving_follow_each_arithmetic_profile(gpu_device, profile):
                     profile=profile)
         assert np.array_equal(outs[k], single, equal_nan=True), k
         assert np.array_equal(served[k].result, single, equal_nan=True), k
+
+
+def test_quick_structure_check_agrees_with_validate():
+    """The batched call's fast structural check of tensor fields accepts a
+    field only when ``validate`` accepts it, with the same boundaries and
+    dtype, and leaves every invalid field to ``validate`` (CPU tensors)."""
+    import torch
+
+    from rlic_amd._lib import validate
+    from rlic_amd.batch import _quick_structure_check
+
+    rng = np.random.default_rng(0)
+    specs = [{"x": "periodic", "y": "closed"}, "closed", {"x": ["closed", "closed"], "y": "periodic"},
+             {"x": "periodic", "y": ("periodic", "closed")}, "bogus", {"x": "closed"}]
+    accepted = rejected = 0
+    for k in range(240):
+        ny, nx = int(rng.integers(1, 12)), int(rng.integers(1, 12))
+        dt = torch.float32 if k % 2 else torch.float64
+        t, u, v = (torch.rand(ny, nx, dtype=dt) for _ in range(3))
+        kern = torch.rand(5, dtype=dt)
+        if k % 7 == 0:
+            u = torch.rand(ny, nx + 1, dtype=dt)
+        if k % 11 == 0:
+            kern = torch.rand(2, 2, dtype=dt)
+        if k % 13 == 0:
+            v = v.to(torch.float16)
+        f = Field(t, u, v, kern, uv_mode=["velocity", "polarization", "sideways"][k % 3],
+                  boundaries=specs[k % len(specs)], iterations=[0, 1, 3, -1][k % 4])
+        quick = _quick_structure_check(f)
+        try:
+            full = validate(f.texture, f.u, f.v, kernel=f.kernel, uv_mode=f.uv_mode,
+                            boundaries=f.boundaries, iterations=f.iterations, check_values=False)
+        except Exception:
+            full = None
+        if quick is None:
+            rejected += 1
+            continue
+        accepted += 1
+        assert full is not None, k
+        assert quick[1] == full[1] and (quick[0].x, quick[0].y) == (full[0].x, full[0].y), k
+    assert accepted > 10 and rejected > 10
