@@ -387,6 +387,33 @@ __global__ __launch_bounds__(kLdsThreads) void lic_replay_lds(const T* __restric
     // otherwise (images smaller than the box) the general modulo
     const bool once_y = i_box0 >= -g.ny && int64_t{i_box0} + bh <= 2 * int64_t{g.ny};
     const bool once_x = j_box0 >= -g.nx && int64_t{j_box0} + bw <= 2 * int64_t{g.nx};
+    // 16-byte copies (lane l fills bytes 16 l .. 16 l + 15 after the row's
+    // LDS base): each lane moves kQuad consecutive elements.  With x 16-byte
+    // aligned and nx, kmid multiples of kQuad, every box column quad starts
+    // at a source column c == 0 (mod kQuad), c + kQuad <= nx (the wrap points
+    // 0 and nx fall between quads), and bw is a multiple of kQuad.
+    constexpr int32_t kQuad = static_cast<int32_t>(16 / sizeof(T));
+    const bool quads = (reinterpret_cast<uintptr_t>(x) & 15u) == 0 && g.nx % kQuad == 0 &&
+                       kmid % kQuad == 0;
+    if (quads) {
+      const int32_t nq = bw / kQuad;  // quads per box row
+      for (int32_t a = __builtin_amdgcn_readfirstlane(wave); a < bh; a += kLdsThreads / 64) {
+        int32_t r = i_box0 + a;
+        r = once_y ? (r < 0 ? r + g.ny : (r >= g.ny ? r - g.ny : r)) : wrap_index(r, g.ny);
+        const T* src_row = x + static_cast<int64_t>(r) * g.nx;
+        for (int32_t q0 = 0; q0 < nq; q0 += 64) {
+          if (q0 + lane < nq) {
+            int32_t c = j_box0 + (q0 + lane) * kQuad;
+            c = once_x ? (c < 0 ? c + g.nx : (c >= g.nx ? c - g.nx : c)) : wrap_index(c, g.nx);
+            __builtin_amdgcn_global_load_lds(
+                (const unsigned __attribute__((address_space(1)))*)(src_row + c),
+                (unsigned __attribute__((address_space(3)))*)(box + (a * stride + q0 * kQuad) *
+                                                                         static_cast<int32_t>(sizeof(T))),
+                16, 0, 0);
+          }
+        }
+      }
+    } else
     for (int32_t a = __builtin_amdgcn_readfirstlane(wave); a < bh; a += kLdsThreads / 64) {
       int32_t r = i_box0 + a;
       r = once_y ? (r < 0 ? r + g.ny : (r >= g.ny ? r - g.ny : r)) : wrap_index(r, g.ny);

@@ -348,3 +348,43 @@ def test_extreme_velocity_magnitudes(profile):
                                    iterations=2, profile=profile)
                 got = _core.convolve_f32(tex, (u, v, uv_mode), k, _bpair(b), 2, profile=profile)
                 _assert_same(got, exp, f"profile={profile} L={klen} {uv_mode} {b}")
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_replay_from_unaligned_output_buffer(dtype, gpu_device):
+    """With an even number of passes, pass 3 reads the caller's output
+    buffer.  The LDS replay stages its box with 16-byte copies only when that
+    buffer is 16-byte aligned (and the row length and kernel half-length are
+    multiples of the copy); an output one element past an aligned address
+    takes the 4-byte copies.  Both must give the oracle's bits."""
+    import ctypes
+
+    import torch
+
+    from rlic_amd import _native
+
+    rng = np.random.default_rng(77)
+    shape, klen = (150, 256), 65           # nx and kmid multiples of 4
+    tex, u, v = _fields(rng, shape, dtype, "holes")
+    k = rng.random(klen).astype(dtype)
+    lib = _native.load()
+    fn = lib.rlic_convolve_f32 if np.dtype(dtype) == np.float32 else lib.rlic_convolve_f64
+    t = [torch.from_numpy(a).to(gpu_device) for a in (tex, u, v, k)]
+    n = tex.size
+    for iterations in (4, 6):
+        exp = cpu.convolve(tex, u, v, kernel=k, boundaries="periodic", iterations=iterations)
+        for shift in (0, 1):
+            store = torch.full((n + 4,), float("nan"), dtype=t[0].dtype, device=gpu_device)
+            out = store[shift:shift + n].view(shape)
+            assert (out.data_ptr() % 16 == 0) == (shift == 0)
+            stream = torch.cuda.current_stream(gpu_device).cuda_stream
+            status = fn(t[0].data_ptr(), shape[1], 1, t[1].data_ptr(), shape[1], 1,
+                        t[2].data_ptr(), shape[1], 1, shape[0], shape[1],
+                        t[3].data_ptr(), klen, 0, 1, 1, 1, 1, iterations, out.data_ptr(),
+                        _native.RLIC_DEVICE, 3, ctypes.c_void_p(stream))
+            _native.raise_for_status(status)
+            torch.cuda.synchronize(gpu_device)
+            _assert_same(out.cpu().numpy(), exp, f"{np.dtype(dtype).name} n={iterations} shift={shift}")
+            # the elements around the output are untouched
+            rest = torch.cat([store[:shift], store[shift + n:]]).cpu().numpy()
+            assert np.isnan(rest).all()
