@@ -1,10 +1,10 @@
 """Summaries of rocprofv3 CSV output for the LIC kernels.
 
-    python -m rlic_amd._profsummary stats  <kernel_stats.csv>
+    python -m rlic_amd._profsummary stats  <kernel_stats.csv> [--templates]
     python -m rlic_amd._profsummary traffic <FETCH_SIZE counter_collection.csv>
                                             <WRITE_SIZE counter_collection.csv>
                                             [--workload NAME] [--out-dir DIR]
-    python -m rlic_amd._profsummary counters <counter_collection.csv> ...
+    python -m rlic_amd._profsummary counters <counter_collection.csv> ... [--templates]
 
 `stats` prints the per-kernel launch count and average duration.
 `traffic` turns the per-dispatch FETCH_SIZE / WRITE_SIZE counters (KiB)
@@ -17,8 +17,11 @@ files that bench.py reports as roofline.traffic.
 given runs: VALU lane utilization SQ_THREAD_CYCLES_VALU /
 (64 * SQ_ACTIVE_INST_VALU), LDS bank-conflict share SQ_LDS_BANK_CONFLICT /
 SQ_LDS_IDX_ACTIVE, L2 hit rate TCC_HIT / (TCC_HIT + TCC_MISS), L1 hit rate
-1 - TCP_TCC_READ_REQ / TCP_TOTAL_ACCESSES, and VALU / LDS instructions per
-wave.
+1 - TCP_TCC_READ_REQ / TCP_TOTAL_ACCESSES, VALU / SALU / LDS / vector-read
+instructions per wave, and the share of wave cycles spent waiting
+SQ_WAIT_INST_ANY / SQ_WAVE_CYCLES.
+With --templates, kernels are told apart by their template arguments
+(e.g. the tile shapes of lic_replay_lds) instead of by name alone.
 """
 
 from __future__ import annotations
@@ -35,17 +38,20 @@ COPY_KERNEL = "lic_copy2d"
 COPY_BYTES = 2**28 * 4  # one lic_copy2d dispatch of the calibration array, each way
 
 
-def short_name(full: str) -> str:
-    """'void rlic::lic_trace<float, ...>(...)' -> 'lic_trace'."""
-    m = re.search(r"rlic::(\w+)", full)
+def short_name(full: str, templates: bool = False) -> str:
+    """'void rlic::lic_trace<float, ...>(...)' -> 'lic_trace' (or, with
+    templates, 'lic_trace<float, ...>')."""
+    m = re.search(r"rlic::(\w+<[^()]*>)" if templates else r"rlic::(\w+)", full)
+    if m is None and templates:
+        m = re.search(r"rlic::(\w+)", full)
     return m.group(1) if m else full.split("(")[0][:60]
 
 
-def read_stats(path: Path) -> dict[str, dict]:
+def read_stats(path: Path, templates: bool = False) -> dict[str, dict]:
     out: dict[str, dict] = {}
     with open(path, newline="") as f:
         for row in csv.DictReader(f):
-            name = short_name(row["Name"])
+            name = short_name(row["Name"], templates)
             agg = out.setdefault(name, {"calls": 0, "total_ns": 0.0})
             agg["calls"] += int(row["Calls"])
             agg["total_ns"] += float(row["TotalDurationNs"])
@@ -96,14 +102,14 @@ def traffic(fetch_csv: Path, write_csv: Path) -> dict[str, dict]:
     return result
 
 
-def counter_ratios(paths: list[Path]) -> dict[str, dict]:
+def counter_ratios(paths: list[Path], templates: bool = False) -> dict[str, dict]:
     """Per kernel: counter totals over all dispatches and derived ratios."""
     totals: dict[str, dict[str, float]] = defaultdict(lambda: defaultdict(float))
     dispatches: dict[str, set] = defaultdict(set)
     for path in paths:
         with open(path, newline="") as f:
             for row in csv.DictReader(f):
-                name = short_name(row["Kernel_Name"])
+                name = short_name(row["Kernel_Name"], templates)
                 counter = row["Counter_Name"].removesuffix("_sum")
                 totals[name][counter] += float(row["Counter_Value"])
                 dispatches[name].add((str(path), row["Dispatch_Id"]))
@@ -119,9 +125,11 @@ def counter_ratios(paths: list[Path]) -> dict[str, dict]:
         if c.get("TCP_TOTAL_ACCESSES"):
             r["l1_hit_rate"] = 1 - c["TCP_TCC_READ_REQ"] / c["TCP_TOTAL_ACCESSES"]
         if c.get("SQ_WAVES"):
-            for k in ("SQ_INSTS_VALU", "SQ_INSTS_LDS"):
+            for k in ("SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS", "SQ_INSTS_VMEM_RD"):
                 if k in c:
                     r[k.lower().replace("sq_insts_", "") + "_per_wave"] = c[k] / c["SQ_WAVES"]
+        if c.get("SQ_WAVE_CYCLES") and "SQ_WAIT_INST_ANY" in c:
+            r["wait_share"] = c["SQ_WAIT_INST_ANY"] / c["SQ_WAVE_CYCLES"]
         out[name] = {"ratios": r, "totals": dict(c), "dispatches": len(dispatches[name])}
     return out
 
@@ -131,6 +139,7 @@ def main(argv: list[str] | None = None) -> int:
     sub = ap.add_subparsers(dest="cmd", required=True)
     s = sub.add_parser("stats")
     s.add_argument("csv", type=Path)
+    s.add_argument("--templates", action="store_true")
     t = sub.add_parser("traffic")
     t.add_argument("fetch_csv", type=Path)
     t.add_argument("write_csv", type=Path)
@@ -138,16 +147,18 @@ def main(argv: list[str] | None = None) -> int:
     t.add_argument("--out-dir", type=Path)
     c = sub.add_parser("counters")
     c.add_argument("csv", type=Path, nargs="+")
+    c.add_argument("--templates", action="store_true")
     args = ap.parse_args(argv)
     if args.cmd == "counters":
-        for name, r in sorted(counter_ratios(args.csv).items()):
+        for name, r in sorted(counter_ratios(args.csv, args.templates).items()):
             if not name.startswith("lic_"):
                 continue
             ratios = ", ".join(f"{k} {v:.4g}" for k, v in r["ratios"].items())
-            print(f"{name:24s} {ratios}")
+            print(f"{name:24s} dispatches {r['dispatches']}, {ratios}")
         return 0
     if args.cmd == "stats":
-        for name, agg in sorted(read_stats(args.csv).items(), key=lambda kv: -kv[1]["total_ns"]):
+        for name, agg in sorted(read_stats(args.csv, args.templates).items(),
+                                key=lambda kv: -kv[1]["total_ns"]):
             print(f"{name:24s} calls {agg['calls']:6d}  avg {agg['avg_ms']:10.4f} ms  "
                   f"total {agg['total_ns'] / 1e6:10.3f} ms")
         return 0
