@@ -13,7 +13,8 @@ reference's ``src/rlic/_lib.py:37-235`` (pinned by its
 
 Beyond the reference, the arrays may also be PyTorch tensors on a ROCm
 device: the same checks then run on the device and the result is a device
-tensor.
+tensor.  Arrays of other libraries that export DLPack (``__dlpack__``) are
+viewed in place as PyTorch tensors (no copy) and handled the same way.
 """
 
 from __future__ import annotations
@@ -43,6 +44,16 @@ _SUPPORTED_DTYPES: list[np.dtype] = [np.dtype("float32"), np.dtype("float64")]
 
 def _is_torch(x: Any) -> bool:
     return type(x).__module__.split(".", 1)[0] == "torch"
+
+
+def _from_dlpack(arr: Any) -> Any:
+    """A zero-copy PyTorch view of a DLPack-exporting array of another
+    library; NumPy arrays, tensors and everything else pass unchanged."""
+    if isinstance(arr, np.ndarray) or _is_torch(arr) or not hasattr(arr, "__dlpack__"):
+        return arr
+    import torch
+
+    return torch.from_dlpack(arr)
 
 
 _TORCH_DTYPES: dict = {}
@@ -148,7 +159,8 @@ def convolve(
     -------
     A new 2D array of the inputs' dtype (float32 or float64).  NumPy inputs
     give a NumPy array; PyTorch tensors on a ROCm device give a tensor on
-    that device.  ``iterations=0`` returns a copy of ``texture``.
+    that device.  Other DLPack-exporting arrays are read in place and give a
+    PyTorch tensor.  ``iterations=0`` returns a copy of ``texture``.
 
     Raises
     ------
@@ -159,6 +171,7 @@ def convolve(
     ExceptionGroup
       When more than one problem is found; all are reported.
     """
+    texture, u, v, kernel = (_from_dlpack(a) for a in (texture, u, v, kernel))
     bs, dtype = validate(
         texture, u, v, kernel=kernel, uv_mode=uv_mode, boundaries=boundaries, iterations=iterations
     )

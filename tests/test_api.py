@@ -290,3 +290,36 @@ def test_multiple_errors():
 def test_allowed_combo_closed_pairs():
     assert COMBO_ALLOWED_BOUNDS == frozenset({"closed"})
     assert not BoundarySet(x=("closed", "closed"), y=("closed", "closed")).collect_exceptions()
+
+
+class _DLPackOnly:
+    """An array of a foreign library: it exports DLPack and nothing else."""
+
+    def __init__(self, arr):
+        self._arr = arr
+
+    def __dlpack__(self, *args, **kwargs):
+        return self._arr.__dlpack__(*args, **kwargs)
+
+    def __dlpack_device__(self):
+        return self._arr.__dlpack_device__()
+
+
+def test_dlpack_producers_are_viewed_in_place():
+    from rlic_amd._lib import _from_dlpack
+
+    a = np.arange(12.0).reshape(3, 4)[:, ::2]  # strided view
+    t = _from_dlpack(_DLPackOnly(a))
+    assert type(t).__module__.startswith("torch")
+    assert t.data_ptr() == a.ctypes.data  # no copy
+    assert tuple(t.stride()) == tuple(s // a.itemsize for s in a.strides)
+    assert _from_dlpack(a) is a
+    plain = [1.0, 2.0]
+    assert _from_dlpack(plain) is plain
+
+
+def test_dlpack_inputs_are_validated():
+    img = _DLPackOnly(np.zeros((4, 4), dtype="int64"))
+    k = _DLPackOnly(np.ones(3, dtype="int64"))
+    with pytest.raises(TypeError, match=r"Found unsupported data type\(s\)"):
+        rlic.convolve(img, img, img, kernel=k)
