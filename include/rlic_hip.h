@@ -237,6 +237,25 @@ int rlic_div_check_f32(const float* a, const float* b, float* q, int64_t n, void
  * cached blocks.  No counterpart in the reference. */
 int rlic_scratch_release(void);
 
+/* On-device synthetic inputs, so that large benchmark and serving inputs
+ * need no host generation or host-to-device copy.  No counterpart in the
+ * reference (its inputs are NumPy arrays).  `out` is ny x nx contiguous
+ * elements of device memory; the calls only enqueue work on `stream`.
+ *
+ * rlic_fill_noise_*: out[p] = uniform in [0, 1) from a counter-based hash
+ * (splitmix64) of `seed` and the element index p, so a seed gives the same
+ * array on any device and at any launch shape.
+ *
+ * rlic_fill_modes_*: out[i][j] = sum over m < n_modes (at most 32) of
+ * amp[m] * cos(2 pi (kx[m] j / nx + ky[m] i / ny) + phase[m]), evaluated in
+ * double; amp, kx, ky and phase are host arrays of n_modes elements. */
+int rlic_fill_noise_f32(float* out, int64_t ny, int64_t nx, uint64_t seed, void* stream);
+int rlic_fill_noise_f64(double* out, int64_t ny, int64_t nx, uint64_t seed, void* stream);
+int rlic_fill_modes_f32(float* out, int64_t ny, int64_t nx, int32_t n_modes, const double* amp,
+                        const int32_t* kx, const int32_t* ky, const double* phase, void* stream);
+int rlic_fill_modes_f64(double* out, int64_t ny, int64_t nx, int32_t n_modes, const double* amp,
+                        const int32_t* kx, const int32_t* ky, const double* phase, void* stream);
+
 /* Library version string.  No counterpart in the reference. */
 const char* rlic_version(void);
 

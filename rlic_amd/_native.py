@@ -99,6 +99,10 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "rlic_scratch_release": ([], _int),
     "rlic_profiler_enable": ([_int], _int),
     "rlic_profiler_read": ([ctypes.c_char_p, _i64], _i64),
+    "rlic_fill_noise_f32": ([_vp, _i64, _i64, ctypes.c_uint64, _vp], _int),
+    "rlic_fill_noise_f64": ([_vp, _i64, _i64, ctypes.c_uint64, _vp], _int),
+    "rlic_fill_modes_f32": ([_vp, _i64, _i64, ctypes.c_int32, _vp, _vp, _vp, _vp, _vp], _int),
+    "rlic_fill_modes_f64": ([_vp, _i64, _i64, ctypes.c_int32, _vp, _vp, _vp, _vp, _vp], _int),
     "rlic_version": ([], ctypes.c_char_p),
 }
 

@@ -769,6 +769,47 @@ int server_read(rlic::Server* server, int64_t handle, T* out) {
 
 }  // namespace
 
+namespace {
+
+template <typename T>
+int fill_noise(T* out, int64_t ny, int64_t nx, uint64_t seed, void* stream) {
+  t_last_error.clear();
+  if (ny < 0 || nx < 0) return set_error(RLIC_ERR_INVALID_ARGUMENT, "negative image size");
+  if (ny > 0 && nx > INT64_MAX / ny) return set_error(RLIC_ERR_INVALID_ARGUMENT, "image too large");
+  if (ny * nx > 0 && !out) return set_error(RLIC_ERR_INVALID_ARGUMENT, "null array pointer");
+  RLIC_HIP_TRY(rlic::launch_fill_noise<T>(out, ny * nx, seed, static_cast<hipStream_t>(stream)),
+               "noise");
+  return RLIC_OK;
+}
+
+template <typename T>
+int fill_modes(T* out, int64_t ny, int64_t nx, int32_t n_modes, const double* amp,
+               const int32_t* kx, const int32_t* ky, const double* phase, void* stream) {
+  t_last_error.clear();
+  if (ny < 0 || nx < 0) return set_error(RLIC_ERR_INVALID_ARGUMENT, "negative image size");
+  if (ny > INT32_MAX / 2 || nx > INT32_MAX / 2)
+    return set_error(RLIC_ERR_INVALID_ARGUMENT, "image too large");
+  if (n_modes < 0 || n_modes > rlic::kMaxModes)
+    return set_error(RLIC_ERR_INVALID_ARGUMENT, "n_modes must be in [0, %d]", rlic::kMaxModes);
+  if (n_modes > 0 && (!amp || !kx || !ky || !phase))
+    return set_error(RLIC_ERR_INVALID_ARGUMENT, "null mode array");
+  if (ny * nx > 0 && !out) return set_error(RLIC_ERR_INVALID_ARGUMENT, "null array pointer");
+  rlic::ModeSet modes{};
+  modes.n = n_modes;
+  for (int32_t m = 0; m < n_modes; ++m) {
+    modes.amp[m] = amp[m];
+    modes.phase[m] = phase[m];
+    modes.kx[m] = kx[m];
+    modes.ky[m] = ky[m];
+  }
+  RLIC_HIP_TRY(rlic::launch_fill_modes<T>(out, static_cast<int32_t>(ny), static_cast<int32_t>(nx),
+                                          modes, static_cast<hipStream_t>(stream)),
+               "Fourier modes");
+  return RLIC_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int rlic_convolve_f32(const float* texture, int64_t tex_sy, int64_t tex_sx, const float* u,
@@ -953,6 +994,24 @@ int rlic_div_check_f32(const float* a, const float* b, float* q, int64_t n, void
   RLIC_HIP_TRY(rlic::launch_div_check(a, b, q, n, s), "division check");
   RLIC_HIP_TRY(hipStreamSynchronize(s), "division check");
   return RLIC_OK;
+}
+
+int rlic_fill_noise_f32(float* out, int64_t ny, int64_t nx, uint64_t seed, void* stream) {
+  return fill_noise<float>(out, ny, nx, seed, stream);
+}
+
+int rlic_fill_noise_f64(double* out, int64_t ny, int64_t nx, uint64_t seed, void* stream) {
+  return fill_noise<double>(out, ny, nx, seed, stream);
+}
+
+int rlic_fill_modes_f32(float* out, int64_t ny, int64_t nx, int32_t n_modes, const double* amp,
+                        const int32_t* kx, const int32_t* ky, const double* phase, void* stream) {
+  return fill_modes<float>(out, ny, nx, n_modes, amp, kx, ky, phase, stream);
+}
+
+int rlic_fill_modes_f64(double* out, int64_t ny, int64_t nx, int32_t n_modes, const double* amp,
+                        const int32_t* kx, const int32_t* ky, const double* phase, void* stream) {
+  return fill_modes<double>(out, ny, nx, n_modes, amp, kx, ky, phase, stream);
 }
 
 int rlic_scratch_release(void) {

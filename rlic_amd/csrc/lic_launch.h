@@ -48,6 +48,20 @@ hipError_t launch_replay(const T* x, const T* kernel, int32_t klen, const FieldG
                          const TileMap& m, int profile, T* out, const uint4* codes,
                          hipStream_t stream);
 
+// Synthetic inputs (lic_generate.hip): uniform noise in [0, 1) for n
+// contiguous elements, and sums of plane waves on an ny x nx grid.
+constexpr int32_t kMaxModes = 32;
+struct ModeSet {
+  int32_t n;                                  // modes in use (<= kMaxModes)
+  double amp[kMaxModes], phase[kMaxModes];    // amplitude, phase (radians)
+  double kx[kMaxModes], ky[kMaxModes];        // integer wavenumbers, as doubles
+};
+template <typename T>
+hipError_t launch_fill_noise(T* out, int64_t n, uint64_t seed, hipStream_t stream);
+template <typename T>
+hipError_t launch_fill_modes(T* out, int32_t ny, int32_t nx, const ModeSet& modes,
+                             hipStream_t stream);
+
 // q[i] = a[i] / b[i] with the streamline kernels' float division (a check).
 hipError_t launch_div_check(const float* a, const float* b, float* q, int64_t n,
                             hipStream_t stream);

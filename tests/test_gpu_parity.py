@@ -388,3 +388,29 @@ def test_replay_from_unaligned_output_buffer(dtype, gpu_device):
             # the elements around the output are untouched
             rest = torch.cat([store[:shift], store[shift + n:]]).cpu().numpy()
             assert np.isnan(rest).all()
+
+
+def test_dlpack_device_arrays(gpu_device):
+    """Device arrays of another library that exports DLPack are read in
+    place; the result is a device tensor with the oracle's bits."""
+    import torch
+
+    class DLPackOnly:
+        def __init__(self, t):
+            self._t = t
+
+        def __dlpack__(self, *args, **kwargs):
+            return self._t.__dlpack__(*args, **kwargs)
+
+        def __dlpack_device__(self):
+            return self._t.__dlpack_device__()
+
+    rng = np.random.default_rng(91)
+    tex, u, v = _fields(rng, (70, 90), np.float32, "holes")
+    k = rng.random(33).astype(np.float32)
+    exp = cpu.convolve(tex, u, v, kernel=k, boundaries="periodic", iterations=3)
+    t = [torch.from_numpy(a).to(gpu_device) for a in (tex, u, v, k)]
+    got = rlic_amd.convolve(*(DLPackOnly(a) for a in t[:3]), kernel=DLPackOnly(t[3]),
+                            boundaries="periodic", iterations=3)
+    assert isinstance(got, torch.Tensor) and got.device == gpu_device
+    _assert_same(got.cpu().numpy(), exp, "dlpack")
