@@ -414,3 +414,19 @@ def test_dlpack_device_arrays(gpu_device):
                             boundaries="periodic", iterations=3)
     assert isinstance(got, torch.Tensor) and got.device == gpu_device
     _assert_same(got.cpu().numpy(), exp, "dlpack")
+
+
+@pytest.mark.parametrize("klen", [33, 34, 35, 36, 40, 66])
+def test_lds_box_kernel_lengths(klen):
+    """LDS replay with kernel half-lengths 16 to 33, odd and even lengths
+    (an even length has one forward step fewer than backward steps):
+    straight lines (unit fields) reach the box's outer rows and columns;
+    both dtypes, with 4-byte and 16-byte box copies (odd and even kmid)."""
+    rng = np.random.default_rng(klen)
+    for dtype in (np.float32, np.float64):
+        for kind in ("ties", "random"):
+            tex, u, v = _fields(rng, (150, 200), dtype, kind)
+            k = rng.random(klen).astype(dtype)
+            exp = cpu.convolve(tex, u, v, kernel=k, boundaries="periodic", iterations=3)
+            got = rlic_amd.convolve(tex, u, v, kernel=k, boundaries="periodic", iterations=3)
+            _assert_same(got, exp, f"{np.dtype(dtype).name} {kind} L={klen}")
