@@ -375,9 +375,10 @@ __global__ __launch_bounds__(kLdsThreads) void lic_replay_lds(const T* __restric
   }
   const int32_t wave = tid >> 6, lane = tid & 63;
   // Stage the box with direct global -> LDS loads (no registers, all in
-  // flight at once): wave w copies rows w, w + 16, ... in 256-byte
-  // segments, lane l one dword of each (coalesced along x, wrapped modulo
-  // the image size per element).
+  // flight at once): wave w copies rows w, w + 16, ..., each row in
+  // segments of 64 lanes, lane l 16 bytes of a segment when the copy below
+  // allows it, else one dword (coalesced along x, wrapped modulo the image
+  // size per element or per 16-byte quad).
   {
     constexpr int32_t kDw = static_cast<int32_t>(sizeof(T) / 4);  // dwords per element
     constexpr int32_t kSegElems = 64 / kDw;                        // elements per segment
