@@ -148,8 +148,10 @@ int env_int(const char* name, int dflt) {
   return std::atoi(s);
 }
 
-// How a call's passes 2..n replay: LDS-staged with a th x tw tile, or L2
-// gathers over a tile map (strip height `band`, wave shape).
+// How a call's passes 2..n replay: lds = 1, LDS-staged with a th x tw tile;
+// lds = 2, the hybrid (64 x 64 tiles, the first steps of each direction
+// from an LDS box, the rest from the image); lds = 0, L2 gathers over a
+// tile map (strip height `band`, wave shape).
 struct ReplayPlan {
   int32_t lds, th, tw, band, wave_w_log2;
 };
@@ -158,6 +160,8 @@ template <typename T>
 hipError_t launch_plan(const ReplayPlan& p, const T* src, T* dst, const T* kernel, int32_t klen,
                        const rlic::FieldGeom& g, int profile, const uint4* codes,
                        hipStream_t stream) {
+  if (p.lds == 2)
+    return rlic::launch_replay_lds_hybrid<T>(src, kernel, klen, g, profile, dst, codes, stream);
   if (p.lds)
     return rlic::launch_replay_lds_tile<T>(src, kernel, klen, g, rlic::LdsTile{p.th, p.tw},
                                            profile, dst, codes, stream);
@@ -168,11 +172,19 @@ hipError_t launch_plan(const ReplayPlan& p, const T* src, T* dst, const T* kerne
 const char* plan_kernel_name(const ReplayPlan& p) { return p.lds ? "lic_replay_lds" : "lic_replay"; }
 
 // Default plan: the largest LDS tile whose box fits (RLIC_REPLAY_LDS=0: none),
-// else L2 gathers over the tracing pass's tile map.
+// else the hybrid when it fits (RLIC_REPLAY_HYBRID=0: never), else L2
+// gathers over the tracing pass's tile map.
 template <typename T>
-ReplayPlan default_plan(int32_t klen) {
+bool hybrid_allowed(int32_t klen, int64_t n) {
+  return env_int("RLIC_REPLAY_LDS", 1) != 0 && env_int("RLIC_REPLAY_HYBRID", 1) != 0 &&
+         rlic::lds_hybrid_fits<T>(klen, n);
+}
+
+template <typename T>
+ReplayPlan default_plan(int32_t klen, int64_t n) {
   const rlic::LdsTile t = rlic::lds_replay_tile<T>(klen);
   if (env_int("RLIC_REPLAY_LDS", 1) != 0 && t.h > 0) return {1, t.h, t.w, 0, 0};
+  if (hybrid_allowed<T>(klen, n)) return {2, 64, 64, 0, 0};
   return {0, 0, 0, env_int("RLIC_TILE_BAND", 16), env_int("RLIC_WAVE_W_LOG2", 3)};
 }
 
@@ -225,7 +237,8 @@ void tuner_load_locked() {
 
 // A plan from the table is used only if it can run here.
 template <typename T>
-bool plan_usable(const ReplayPlan& p, int32_t klen) {
+bool plan_usable(const ReplayPlan& p, int32_t klen, int64_t n) {
+  if (p.lds == 2) return p.th == 64 && p.tw == 64 && rlic::lds_hybrid_fits<T>(klen, n);
   if (p.lds) {
     const rlic::LdsTile big = rlic::lds_replay_tile<T>(klen);
     const bool shape = (p.th == 64 && p.tw == 64) || (p.th == 32 && (p.tw == 64 || p.tw == 32));
@@ -248,7 +261,7 @@ void tuner_store_locked(const TuneKey& k, const TuneEntry& e) {
 }
 
 template <typename T>
-std::vector<ReplayPlan> replay_candidates(int32_t klen) {
+std::vector<ReplayPlan> replay_candidates(int32_t klen, int64_t n) {
   std::vector<ReplayPlan> c;
   const rlic::LdsTile big = rlic::lds_replay_tile<T>(klen);
   if (env_int("RLIC_REPLAY_LDS", 1) != 0 && big.h > 0) {
@@ -256,6 +269,7 @@ std::vector<ReplayPlan> replay_candidates(int32_t klen) {
     if (big.h == 64 || big.w == 64) c.push_back({1, 32, 64, 0, 0});
     c.push_back({1, 32, 32, 0, 0});
   }
+  if (hybrid_allowed<T>(klen, n)) c.push_back({2, 64, 64, 0, 0});
   c.push_back({0, 0, 0, 16, 3});
   c.push_back({0, 0, 0, 4, 3});
   return c;
@@ -497,7 +511,7 @@ int run_on_device(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
     }
     if (iterations == 1) return RLIC_OK;  // recorded only for the statistics
     // the replay plan: tuned (see Tuner), or the default
-    ReplayPlan plan = default_plan<T>(klen);
+    ReplayPlan plan = default_plan<T>(klen, n);
     int64_t first = 2;
     if (env_int("RLIC_AUTOTUNE", 1) != 0 && iterations >= 3 && n >= (int64_t{1} << 18)) {
       const TuneKey key{static_cast<int32_t>(sizeof(T)), ny, nx, klen, profile};
@@ -505,9 +519,9 @@ int run_on_device(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
       tuner_load_locked();
       auto hit = g_tuner.table.find(key);
       if (hit != g_tuner.table.end()) {
-        if (plan_usable<T>(hit->second.plan, klen)) plan = hit->second.plan;
+        if (plan_usable<T>(hit->second.plan, klen, n)) plan = hit->second.plan;
       } else {
-        const std::vector<ReplayPlan> cands = replay_candidates<T>(klen);
+        const std::vector<ReplayPlan> cands = replay_candidates<T>(klen, n);
         TuneEntry best{plan, -1.f};
         RLIC_HIP_TRY(tune_replay<T>(cands, a, b, kernel, klen, g, profile, codes, stream, best),
                      "tuning the replay");
@@ -1034,7 +1048,8 @@ int64_t rlic_autotune_read(char* buf, int64_t len) {
              "\"replay\": \"%s\", \"tile\": [%d, %d], \"band\": %d, \"wave_w_log2\": %d, "
              "\"pass_ms\": %.6f}",
              first ? "" : ", ", k.dtype_size, k.ny, k.nx, k.klen, k.profile,
-             e.plan.lds ? "lds" : "l2", e.plan.th, e.plan.tw, e.plan.band, e.plan.wave_w_log2,
+             e.plan.lds == 2 ? "lds_hybrid" : (e.plan.lds ? "lds" : "l2"), e.plan.th, e.plan.tw,
+             e.plan.band, e.plan.wave_w_log2,
              e.ms);
     json += item;
     first = false;

@@ -195,23 +195,35 @@ __global__ __launch_bounds__(256) void lic_replay(const T* __restrict__ xin,
 // ---------------------------------------------------------------------------
 constexpr int32_t kLdsThreads = 1024;
 
-__host__ __device__ inline int32_t lds_box_h(int32_t th, int32_t klen) { return th + 2 * (klen / 2); }
-__host__ __device__ inline int32_t lds_box_w(int32_t tw, int32_t klen) { return tw + 2 * (klen / 2); }
-__host__ __device__ inline int32_t lds_box_stride(int32_t tw, int32_t klen) {
-  const int32_t w = lds_box_w(tw, klen);
+// Box of a tile grown by `halo` cells on each side (halo = kmid: every
+// step of every line; smaller: the first `halo` steps, see the hybrid
+// replay below).
+__host__ __device__ inline int32_t lds_box_h(int32_t th, int32_t halo) { return th + 2 * halo; }
+__host__ __device__ inline int32_t lds_box_w(int32_t tw, int32_t halo) { return tw + 2 * halo; }
+__host__ __device__ inline int32_t lds_box_stride(int32_t tw, int32_t halo) {
+  const int32_t w = lds_box_w(tw, halo);
   return w + ((8 - w) % 32 + 32) % 32;  // == 8 (mod 32)
 }
 
-constexpr size_t kLdsPairTableBytes = 256 * sizeof(int2);  // static LDS
+constexpr size_t kLdsPairTableBytes = 256 * sizeof(int2);  // static LDS, per table
 
 // Dynamic LDS of one block: the box.
-inline size_t lds_replay_dynamic_bytes(int32_t th, int32_t tw, int32_t klen, size_t elem) {
-  return static_cast<size_t>(lds_box_h(th, klen)) * lds_box_stride(tw, klen) * elem;
+inline size_t lds_replay_dynamic_bytes(int32_t th, int32_t tw, int32_t halo, size_t elem) {
+  return static_cast<size_t>(lds_box_h(th, halo)) * lds_box_stride(tw, halo) * elem;
 }
 
-// All LDS of one block.
-inline size_t lds_replay_bytes(int32_t th, int32_t tw, int32_t klen, size_t elem) {
-  return kLdsPairTableBytes + lds_replay_dynamic_bytes(th, tw, klen, elem);
+// All LDS of one block (`tables` pair tables: 1, or 2 for the hybrid replay).
+inline size_t lds_replay_bytes(int32_t th, int32_t tw, int32_t halo, size_t elem,
+                               int32_t tables = 1) {
+  return tables * kLdsPairTableBytes + lds_replay_dynamic_bytes(th, tw, halo, elem);
+}
+
+// Hybrid replay: a 64 x 64 tile whose box holds the first kHybridHalo steps
+// of each direction (a multiple of the 32-step chunk); later steps gather
+// from the image through buffer loads.
+template <typename T>
+constexpr int32_t hybrid_halo() {
+  return sizeof(T) == 4 ? 64 : 32;
 }
 
 __device__ __forceinline__ int32_t wrap_index(int32_t c, int32_t n) {
@@ -231,9 +243,10 @@ __device__ __forceinline__ int32_t lds_move(uint32_t c, const FieldGeom& g, int3
   return neg ? -d : d;
 }
 
-// The 8 values one line reads over a group of 8 moves (`word`, one 4-bit
-// move code per step), advancing its pointer `at` into the box.  The
-// pointer (an LDS address) is the line's state, so each step is one add.
+// A line's position while replaying: an LDS byte pointer into the staged
+// box (LdsWalk) or a byte offset into the image, read with buffer loads
+// (ImageWalk, the hybrid replay's steps past the box).  Either way each
+// step is one add of a pair-table delta.
 typedef const unsigned char __attribute__((address_space(3))) lds_byte;
 
 template <typename T>
@@ -242,35 +255,72 @@ __device__ __forceinline__ T lds_value(lds_byte* at) {
   return *reinterpret_cast<lds_t*>(at);
 }
 
+// kTableFirst: the replay loop reads the pair-table entries of all of a
+// lane's lines before their gathers (LDS gathers are short; the early table
+// reads hide their latency), or each line's just before its own gathers
+// (image gathers are long; fewer registers live).
 template <typename T>
-__device__ __forceinline__ void lds_group(T (&xv)[8], lds_byte*& at, uint32_t word,
-                                          const int2* pair_delta) {
+struct LdsWalk {
+  typedef lds_byte* Pos;
+  static constexpr bool kTableFirst = true;
+  __device__ __forceinline__ T operator()(Pos at) const { return lds_value<T>(at); }
+};
+
+template <typename T>
+struct ImageWalk {
+  typedef uint32_t Pos;
+  static constexpr bool kTableFirst = false;
+  __amdgpu_buffer_rsrc_t rsrc;  // the image, n elements with n * sizeof(T) < 2^32
+  // base and size pass through readfirstlane: the descriptor must be
+  // wave-uniform (scalar registers), which the compiler does not prove here
+  __device__ __forceinline__ ImageWalk(const T* base, int64_t n) {
+    const uint64_t b = reinterpret_cast<uint64_t>(base);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(b));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(b >> 32));
+    const uint32_t bytes =
+        __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(n * static_cast<int64_t>(sizeof(T))));
+    rsrc = make_rsrc(reinterpret_cast<const void*>((uint64_t{hi} << 32) | lo), bytes);
+  }
+  __device__ __forceinline__ T operator()(Pos at) const {
+    return buf_load(rsrc, at, static_cast<T*>(nullptr));
+  }
+};
+
+// The 8 values one line reads over a group of 8 moves (`word`, one 4-bit
+// move code per step), advancing its position `at`.
+template <typename T, typename Walk>
+__device__ __forceinline__ void walk_group(const Walk& walk, T (&xv)[8],
+                                           typename Walk::Pos& at, uint32_t word,
+                                           const int2* pair_delta) {
 #pragma unroll
   for (int h = 0; h < 4; ++h) {
     const int2 d = pair_delta[(word >> (8 * h)) & 255u];
     at += d.x;
-    xv[2 * h] = lds_value<T>(at);
+    xv[2 * h] = walk(at);
     at += d.y;
-    xv[2 * h + 1] = lds_value<T>(at);
+    xv[2 * h + 1] = walk(at);
   }
 }
 
-template <typename T, bool FMA, bool FORWARD, int E>
-__device__ __forceinline__ void replay_directions_lds(
-    T (&acc)[E], lds_byte* const (&at0)[E], const bool (&inside)[E],
+// Replays chunks [ch_begin, min(ch_end, chunks of nsteps)) of one direction
+// of E lines per lane; `at` and `alive` carry the lines' state in and out.
+template <typename T, bool FMA, bool FORWARD, int E, typename Walk>
+__device__ __forceinline__ void replay_lines(
+    const Walk& walk, T (&acc)[E], typename Walk::Pos (&at)[E], bool (&alive)[E],
     const T* __restrict__ kernel, int32_t kmid, int32_t nsteps, const int2* pair_delta,
-    const uint4* __restrict__ codes, int64_t np, const int64_t (&p)[E], int32_t c0) {
-  const int32_t nchunks = chunks_of(nsteps);
-  bool alive[E];
-  lds_byte* at[E];
+    const uint4* __restrict__ codes, int64_t np, const int64_t (&p)[E], int32_t c0,
+    int32_t ch_begin, int32_t ch_end) {
+  const int32_t nchunks = min(chunks_of(nsteps), ch_end);
   uint4 next[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    alive[e] = inside[e];
-    at[e] = at0[e];
-    next[e] = (nchunks > 0 && alive[e]) ? codes[c0 * np + p[e]] : make_uint4(0u, 0u, 0u, 0u);
-  }
-  for (int32_t ch = 0; ch < nchunks; ++ch) {
+  for (int e = 0; e < E; ++e)
+    next[e] = (ch_begin < nchunks && alive[e]) ? codes[(c0 + ch_begin) * np + p[e]]
+                                               : make_uint4(0u, 0u, 0u, 0u);
+  bool any_line = false;
+#pragma unroll
+  for (int e = 0; e < E; ++e) any_line = any_line || alive[e];
+  if (__builtin_amdgcn_ballot_w64(any_line) == 0) return;
+  for (int32_t ch = ch_begin; ch < nchunks; ++ch) {
     uint32_t words[E][4];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -299,23 +349,29 @@ __device__ __forceinline__ void replay_directions_lds(
       if (__builtin_amdgcn_ballot_w64(!full) == 0) {
         // every lane accumulates all 8 steps of every pixel (wave-uniform):
         // no per-step masks, and every line stays alive.  The pair-table
-        // reads of all E lines go first, and line e + 1's gathers are
+        // reads go first (see kTableFirst), and line e + 1's gathers are
         // issued before line e accumulates, so gathers overlap.
         int2 d[E][4];
+        if constexpr (Walk::kTableFirst) {
 #pragma unroll
-        for (int e = 0; e < E; ++e)
+          for (int e = 0; e < E; ++e)
 #pragma unroll
-          for (int h = 0; h < 4; ++h) d[e][h] = pair_delta[(word[e] >> (8 * h)) & 255u];
+            for (int h = 0; h < 4; ++h) d[e][h] = pair_delta[(word[e] >> (8 * h)) & 255u];
+        }
         T xv[2][8];
 #pragma unroll
         for (int e = 0; e <= E; ++e) {
           if (e < E) {
+            if constexpr (!Walk::kTableFirst) {
+#pragma unroll
+              for (int h = 0; h < 4; ++h) d[e][h] = pair_delta[(word[e] >> (8 * h)) & 255u];
+            }
 #pragma unroll
             for (int h = 0; h < 4; ++h) {
               at[e] += d[e][h].x;
-              xv[e & 1][2 * h] = lds_value<T>(at[e]);
+              xv[e & 1][2 * h] = walk(at[e]);
               at[e] += d[e][h].y;
-              xv[e & 1][2 * h + 1] = lds_value<T>(at[e]);
+              xv[e & 1][2 * h + 1] = walk(at[e]);
             }
           }
           if (e > 0) {
@@ -330,7 +386,7 @@ __device__ __forceinline__ void replay_directions_lds(
           const uint32_t stops = stop_nibbles(word[e]);
           const int32_t valid = alive[e] ? (stops ? (__builtin_ctz(stops) >> 2) : 8) : 0;
           T xv[8];
-          lds_group<T>(xv, at[e], word[e], pair_delta);
+          walk_group<T>(walk, xv, at[e], word[e], pair_delta);
           // branch-free over the group (see replay_direction)
           const int32_t use = min(valid, lim);
 #pragma unroll
@@ -349,29 +405,39 @@ __device__ __forceinline__ void replay_directions_lds(
   }
 }
 
-template <typename T, bool FMA, int TH, int TW>
+template <typename T, bool FMA, int TH, int TW, bool HYBRID>
 __global__ __launch_bounds__(kLdsThreads) void lic_replay_lds(const T* __restrict__ x,
                                                               const T* __restrict__ kernel,
                                                               int32_t klen, FieldGeom g,
                                                               TileMap m, T* __restrict__ out,
                                                               const uint4* __restrict__ codes) {
   constexpr int E = TH * TW / kLdsThreads;  // pixels per lane: 1, 2 or 4
+  constexpr int32_t elem = static_cast<int32_t>(sizeof(T));
   extern __shared__ __align__(16) unsigned char smem[];
   const int32_t kmid = klen / 2;
-  const int32_t bh = lds_box_h(TH, klen), bw = lds_box_w(TW, klen);
-  const int32_t stride = lds_box_stride(TW, klen);
+  // the box holds every step (halo kmid), or with HYBRID the first
+  // hybrid_halo() steps of each direction (the launcher picks HYBRID only
+  // when kmid is larger)
+  const int32_t halo = HYBRID ? hybrid_halo<T>() : kmid;
+  const int32_t bh = lds_box_h(TH, halo), bw = lds_box_w(TW, halo);
+  const int32_t stride = lds_box_stride(TW, halo);
   __shared__ int2 pair_delta[256];
+  __shared__ int2 pair_image[HYBRID ? 256 : 1];  // image-offset deltas (wraps jump)
   unsigned char* box = smem;
 
   int32_t ty, tx;
   tile_of_block(blockIdx.x, m, ty, tx);
-  const int32_t i_box0 = ty * TH - kmid, j_box0 = tx * TW - kmid;
+  const int32_t i_box0 = ty * TH - halo, j_box0 = tx * TW - halo;
   const int32_t tid = threadIdx.x;
-  const int32_t row_bytes = stride * static_cast<int32_t>(sizeof(T));
+  const int32_t row_bytes = stride * elem;
   if (tid < 256) {
-    const int32_t lo = lds_move(tid & 15, g, row_bytes, sizeof(T));
-    const int32_t hi = lds_move(tid >> 4, g, row_bytes, sizeof(T));
+    const int32_t lo = lds_move(tid & 15, g, row_bytes, elem);
+    const int32_t hi = lds_move(tid >> 4, g, row_bytes, elem);
     pair_delta[tid] = make_int2(lo, hi);
+    if constexpr (HYBRID) {
+      pair_image[tid] = make_int2(static_cast<int32_t>(move_delta(tid & 15, g, elem)),
+                                  static_cast<int32_t>(move_delta(tid >> 4, g, elem)));
+    }
   }
   const int32_t wave = tid >> 6, lane = tid & 63;
   // Stage the box with direct global -> LDS loads (no registers, all in
@@ -390,12 +456,12 @@ __global__ __launch_bounds__(kLdsThreads) void lic_replay_lds(const T* __restric
     const bool once_x = j_box0 >= -g.nx && int64_t{j_box0} + bw <= 2 * int64_t{g.nx};
     // 16-byte copies (lane l fills bytes 16 l .. 16 l + 15 after the row's
     // LDS base): each lane moves kQuad consecutive elements.  With x 16-byte
-    // aligned and nx, kmid multiples of kQuad, every box column quad starts
+    // aligned and nx, halo multiples of kQuad, every box column quad starts
     // at a source column c == 0 (mod kQuad), c + kQuad <= nx (the wrap points
     // 0 and nx fall between quads), and bw is a multiple of kQuad.
     constexpr int32_t kQuad = static_cast<int32_t>(16 / sizeof(T));
     const bool quads = (reinterpret_cast<uintptr_t>(x) & 15u) == 0 && g.nx % kQuad == 0 &&
-                       kmid % kQuad == 0;
+                       halo % kQuad == 0;
     if (quads) {
       const int32_t nq = bw / kQuad;  // quads per box row
       for (int32_t a = __builtin_amdgcn_readfirstlane(wave); a < bh; a += kLdsThreads / 64) {
@@ -453,13 +519,73 @@ __global__ __launch_bounds__(kLdsThreads) void lic_replay_lds(const T* __restric
     const int32_t i0 = ty * TH + py, j0 = tx * TW + px;
     inside[e] = i0 < g.ny && j0 < g.nx;
     p[e] = inside[e] ? static_cast<int64_t>(i0) * g.nx + j0 : 0;
-    at0[e] = (lds_byte*)box + ((kmid + py) * stride + (kmid + px)) * static_cast<int32_t>(sizeof(T));
+    at0[e] = (lds_byte*)box + ((halo + py) * stride + (halo + px)) * elem;
     acc[e] = macc<FMA>(kernel[kmid], lds_value<T>(at0[e]), T(0));
   }
-  replay_directions_lds<T, FMA, true, E>(acc, at0, inside, kernel, kmid, n_fwd,
-                                         pair_delta, codes, np, p, 0);
-  replay_directions_lds<T, FMA, false, E>(acc, at0, inside, kernel, kmid, n_bwd,
-                                          pair_delta, codes, np, p, chunks_of(n_fwd));
+  // steps inside the box, then (HYBRID) the rest from the image: the box
+  // position becomes an image position, wrapped (a periodic edge is just
+  // crossed in the box); closed edges never leave the image (their clamps
+  // were recorded as "stay")
+  const int32_t ch_box = HYBRID ? halo / kChunkSteps : INT32_MAX;
+#pragma unroll
+  for (int dir = 0; dir < 2; ++dir) {
+    const int32_t nsteps = dir == 0 ? n_fwd : n_bwd;
+    const int32_t c0 = dir == 0 ? 0 : chunks_of(n_fwd);
+    lds_byte* at[E];
+    bool alive[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      at[e] = at0[e];
+      alive[e] = inside[e];
+    }
+    if (dir == 0) {
+      replay_lines<T, FMA, true, E>(LdsWalk<T>{}, acc, at, alive, kernel, kmid, nsteps,
+                                    pair_delta, codes, np, p, c0, 0, ch_box);
+    } else {
+      replay_lines<T, FMA, false, E>(LdsWalk<T>{}, acc, at, alive, kernel, kmid, nsteps,
+                                     pair_delta, codes, np, p, c0, 0, ch_box);
+    }
+    if constexpr (HYBRID) {
+      if (nsteps > halo) {  // uniform
+        // built here, next to its use, so that its descriptor stays in
+        // scalar registers
+        const ImageWalk<T> image(x, np);
+        uint32_t pos[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int32_t cell = static_cast<int32_t>(at[e] - (lds_byte*)box) / elem;
+          const int32_t r = cell / stride, c = cell - r * stride;
+          const int32_t i = wrap_index(i_box0 + r, g.ny), j = wrap_index(j_box0 + c, g.nx);
+          pos[e] = static_cast<uint32_t>(static_cast<int64_t>(i) * g.nx + j) * elem;
+        }
+        // two lines per lane at a time: half the registers of E
+        constexpr int E2 = E >= 2 ? 2 : 1;
+#pragma unroll
+        for (int e0 = 0; e0 < E; e0 += E2) {
+          T acc2[E2];
+          uint32_t pos2[E2];
+          bool alive2[E2];
+          int64_t p2[E2];
+#pragma unroll
+          for (int k = 0; k < E2; ++k) {
+            acc2[k] = acc[e0 + k];
+            pos2[k] = pos[e0 + k];
+            alive2[k] = alive[e0 + k];
+            p2[k] = p[e0 + k];
+          }
+          if (dir == 0) {
+            replay_lines<T, FMA, true, E2>(image, acc2, pos2, alive2, kernel, kmid, nsteps,
+                                           pair_image, codes, np, p2, c0, ch_box, INT32_MAX);
+          } else {
+            replay_lines<T, FMA, false, E2>(image, acc2, pos2, alive2, kernel, kmid, nsteps,
+                                            pair_image, codes, np, p2, c0, ch_box, INT32_MAX);
+          }
+#pragma unroll
+          for (int k = 0; k < E2; ++k) acc[e0 + k] = acc2[k];
+        }
+      }
+    }
+  }
 #pragma unroll
   for (int e = 0; e < E; ++e)
     if (inside[e]) out[p[e]] = acc[e];
@@ -635,7 +761,7 @@ LdsTile lds_replay_tile(int32_t klen) {
   const size_t limit = static_cast<size_t>(lds_replay_limit_bytes());
   const int32_t shapes[3][2] = {{64, 64}, {32, 64}, {32, 32}};
   for (const auto& sh : shapes) {
-    if (lds_replay_bytes(sh[0], sh[1], klen, sizeof(T)) <= limit) return {sh[0], sh[1]};
+    if (lds_replay_bytes(sh[0], sh[1], klen / 2, sizeof(T)) <= limit) return {sh[0], sh[1]};
   }
   return {0, 0};
 }
@@ -645,7 +771,14 @@ bool lds_replay_fits(int32_t klen) {
   return lds_replay_tile<T>(klen).h > 0;
 }
 
-template <typename T, bool FMA, int TH, int TW>
+template <typename T>
+bool lds_hybrid_fits(int32_t klen, int64_t n) {
+  return klen / 2 > hybrid_halo<T>() && fits_buffer(n * static_cast<int64_t>(sizeof(T))) &&
+         lds_replay_bytes(64, 64, hybrid_halo<T>(), sizeof(T), 2) <=
+             static_cast<size_t>(lds_replay_limit_bytes());
+}
+
+template <typename T, bool FMA, int TH, int TW, bool HYBRID = false>
 static void replay_lds_one(const T* x, const T* kernel, int32_t klen, const FieldGeom& g, T* out,
                            const uint4* codes, hipStream_t stream) {
   TileMap m;
@@ -656,12 +789,13 @@ static void replay_lds_one(const T* x, const T* kernel, int32_t klen, const Fiel
   m.per_xcd = m.n_tiles / kXcds;
   m.rem_xcd = m.n_tiles % kXcds;
   m.band = 4;
-  const size_t bytes = lds_replay_dynamic_bytes(TH, TW, klen, sizeof(T));
-  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&lic_replay_lds<T, FMA, TH, TW>),
+  const int32_t halo = HYBRID ? hybrid_halo<T>() : klen / 2;
+  const size_t bytes = lds_replay_dynamic_bytes(TH, TW, halo, sizeof(T));
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&lic_replay_lds<T, FMA, TH, TW, HYBRID>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   (void)hipGetLastError();  // the attribute is advisory on this target
-  hipLaunchKernelGGL((lic_replay_lds<T, FMA, TH, TW>), dim3(m.n_tiles), dim3(kLdsThreads), bytes,
-                     stream, x, kernel, klen, g, m, out, codes);
+  hipLaunchKernelGGL((lic_replay_lds<T, FMA, TH, TW, HYBRID>), dim3(m.n_tiles), dim3(kLdsThreads),
+                     bytes, stream, x, kernel, klen, g, m, out, codes);
 }
 
 template <typename T, bool FMA>
@@ -707,13 +841,25 @@ template <typename T>
 hipError_t launch_replay_lds_tile(const T* x, const T* kernel, int32_t klen, const FieldGeom& g,
                                   LdsTile tile, int profile, T* out, const uint4* codes,
                                   hipStream_t stream) {
-  if (lds_replay_bytes(tile.h, tile.w, klen, sizeof(T)) >
+  if (lds_replay_bytes(tile.h, tile.w, klen / 2, sizeof(T)) >
       static_cast<size_t>(lds_replay_limit_bytes()))
     return hipErrorInvalidValue;
   const bool ok = (profile & 1)
                       ? replay_lds_tile_fma<T, true>(x, kernel, klen, g, tile, out, codes, stream)
                       : replay_lds_tile_fma<T, false>(x, kernel, klen, g, tile, out, codes, stream);
   return ok ? hipGetLastError() : hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t launch_replay_lds_hybrid(const T* x, const T* kernel, int32_t klen, const FieldGeom& g,
+                                    int profile, T* out, const uint4* codes, hipStream_t stream) {
+  if (!lds_hybrid_fits<T>(klen, static_cast<int64_t>(g.ny) * g.nx)) return hipErrorInvalidValue;
+  if (profile & 1) {
+    replay_lds_one<T, true, 64, 64, true>(x, kernel, klen, g, out, codes, stream);
+  } else {
+    replay_lds_one<T, false, 64, 64, true>(x, kernel, klen, g, out, codes, stream);
+  }
+  return hipGetLastError();
 }
 
 #define RLIC_INSTANTIATE(T)                                                                     \
@@ -733,7 +879,11 @@ hipError_t launch_replay_lds_tile(const T* x, const T* kernel, int32_t klen, con
   template hipError_t launch_replay_lds<T>(const T*, const T*, int32_t, const FieldGeom&, int, \
                                            T*, const uint4*, hipStream_t);                      \
   template hipError_t launch_replay_lds_tile<T>(const T*, const T*, int32_t, const FieldGeom&,  \
-                                                LdsTile, int, T*, const uint4*, hipStream_t);
+                                                LdsTile, int, T*, const uint4*, hipStream_t);  \
+  template bool lds_hybrid_fits<T>(int32_t, int64_t);                                          \
+  template hipError_t launch_replay_lds_hybrid<T>(const T*, const T*, int32_t,                 \
+                                                  const FieldGeom&, int, T*, const uint4*,     \
+                                                  hipStream_t);
 
 RLIC_INSTANTIATE(float)
 RLIC_INSTANTIATE(double)

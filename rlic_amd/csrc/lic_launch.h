@@ -92,4 +92,15 @@ hipError_t launch_replay_lds_tile(const T* x, const T* kernel, int32_t klen, con
                                   LdsTile tile, int profile, T* out, const uint4* codes,
                                   hipStream_t stream);
 
+// Hybrid replay for kernels whose full box does not fit: 64x64 tiles whose
+// box holds the first 64 (float) or 32 (double) steps of each direction,
+// later steps gathered from the image with buffer loads.  lds_hybrid_fits:
+// kmid exceeds that halo, the box fits and the image (n elements) is
+// buffer-addressable; otherwise the launch returns hipErrorInvalidValue.
+template <typename T>
+bool lds_hybrid_fits(int32_t klen, int64_t n);
+template <typename T>
+hipError_t launch_replay_lds_hybrid(const T* x, const T* kernel, int32_t klen, const FieldGeom& g,
+                                    int profile, T* out, const uint4* codes, hipStream_t stream);
+
 }  // namespace rlic

@@ -430,3 +430,28 @@ def test_lds_box_kernel_lengths(klen):
             exp = cpu.convolve(tex, u, v, kernel=k, boundaries="periodic", iterations=3)
             got = rlic_amd.convolve(tex, u, v, kernel=k, boundaries="periodic", iterations=3)
             _assert_same(got, exp, f"{np.dtype(dtype).name} {kind} L={klen}")
+
+
+@pytest.mark.parametrize("dtype, klen", [
+    (np.float32, 167), (np.float32, 200), (np.float32, 257),   # hybrid halo 64
+    (np.float64, 111), (np.float64, 150), (np.float64, 257),   # hybrid halo 32
+])
+def test_hybrid_replay(dtype, klen, monkeypatch):
+    """Kernel lengths whose whole LDS box does not fit replay with the
+    hybrid plan: the first 64 (float) or 32 (double) steps of each direction
+    from the LDS box, the rest gathered from the image.  It must give the
+    oracle's bits and the L2-gather replay's, on fields larger and smaller
+    than the box, with NaN holes, periodic and closed sides."""
+    rng = np.random.default_rng(klen)
+    k = rng.random(klen).astype(dtype)
+    for shape in ((150, 200), (40, 33), (130, 3)):
+        for kind in ("holes", "ties"):
+            tex, u, v = _fields(rng, shape, dtype, kind)
+            for b in ("periodic", {"x": "periodic", "y": "closed"}):
+                exp = cpu.convolve(tex, u, v, kernel=k, boundaries=b, iterations=3)
+                monkeypatch.setenv("RLIC_REPLAY_HYBRID", "1")
+                got = rlic_amd.convolve(tex, u, v, kernel=k, boundaries=b, iterations=3)
+                _assert_same(got, exp, f"hybrid {shape} {kind} {b}")
+                monkeypatch.setenv("RLIC_REPLAY_HYBRID", "0")
+                l2 = rlic_amd.convolve(tex, u, v, kernel=k, boundaries=b, iterations=3)
+                _assert_same(l2, exp, f"l2 {shape} {kind} {b}")

@@ -12,9 +12,15 @@ from rlic_amd import _core, _native
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("klen", [33, 301])  # LDS-staged replay, then L2-gather replay
-def test_graph_replays_match_direct_launches(gpu_device, monkeypatch, klen):
+@pytest.mark.parametrize("klen, hybrid, kernel_name", [
+    (33, "1", "lic_replay_lds"),     # LDS-staged replay
+    (301, "1", "lic_replay_lds"),    # hybrid replay (box too large for LDS)
+    (301, "0", "lic_replay"),        # L2-gather replay
+])
+def test_graph_replays_match_direct_launches(gpu_device, monkeypatch, klen, hybrid, kernel_name):
     import torch
+
+    monkeypatch.setenv("RLIC_REPLAY_HYBRID", hybrid)
 
     rng = np.random.default_rng(klen)
     ny, nx = 150, 170
@@ -42,7 +48,7 @@ def test_graph_replays_match_direct_launches(gpu_device, monkeypatch, klen):
         prof = _native.profiler_read()
     finally:
         _native.profiler_enable(False)
-    replay = prof.get("lic_replay_lds" if klen == 33 else "lic_replay")
+    replay = prof.get(kernel_name)
     assert replay is not None and replay["count"] == 5 * 4  # four passes per call
     # the buffers repeat, so most calls replay a cached graph
     assert prof.get("lic_graph_capture", {"count": 0})["count"] < 5
