@@ -171,8 +171,11 @@ hipError_t launch_plan(const ReplayPlan& p, const T* src, T* dst, const T* kerne
 
 const char* plan_kernel_name(const ReplayPlan& p) { return p.lds ? "lic_replay_lds" : "lic_replay"; }
 
-// Default plan: the largest LDS tile whose box fits (RLIC_REPLAY_LDS=0: none),
-// else the hybrid when it fits (RLIC_REPLAY_HYBRID=0: never), else L2
+// Default plan: a 64x64 LDS tile when its box fits (RLIC_REPLAY_LDS=0:
+// never); else, in float, the hybrid when it fits (RLIC_REPLAY_HYBRID=0:
+// never; measured 1.5x faster per pass than the 32x64 and 32x32 boxes at
+// L = 151, 165); else the largest fitting smaller box (in double the
+// hybrid's register spills make those faster); else the hybrid; else L2
 // gathers over the tracing pass's tile map.
 template <typename T>
 bool hybrid_allowed(int32_t klen, int64_t n) {
@@ -183,8 +186,12 @@ bool hybrid_allowed(int32_t klen, int64_t n) {
 template <typename T>
 ReplayPlan default_plan(int32_t klen, int64_t n) {
   const rlic::LdsTile t = rlic::lds_replay_tile<T>(klen);
-  if (env_int("RLIC_REPLAY_LDS", 1) != 0 && t.h > 0) return {1, t.h, t.w, 0, 0};
-  if (hybrid_allowed<T>(klen, n)) return {2, 64, 64, 0, 0};
+  const bool lds = env_int("RLIC_REPLAY_LDS", 1) != 0;
+  if (lds && t.h == 64) return {1, 64, 64, 0, 0};
+  const bool hybrid = hybrid_allowed<T>(klen, n);
+  if (hybrid && sizeof(T) == 4) return {2, 64, 64, 0, 0};
+  if (lds && t.h > 0) return {1, t.h, t.w, 0, 0};
+  if (hybrid) return {2, 64, 64, 0, 0};
   return {0, 0, 0, env_int("RLIC_TILE_BAND", 16), env_int("RLIC_WAVE_W_LOG2", 3)};
 }
 

@@ -287,15 +287,17 @@ def test_replay_paths_match(lds, dtype, monkeypatch):
             _assert_same(got, exp, f"lds={lds} {shape} L={klen} {b}")
 
 
+@pytest.mark.parametrize("hybrid", ["1", "0"])
 @pytest.mark.parametrize("dtype,klen", [
     (np.float32, 65),   # 64x64 LDS tiles
-    (np.float32, 137),  # 32x64 LDS tiles
-    (np.float32, 161),  # 32x32 LDS tiles
-    (np.float32, 181),  # box too large for LDS: global gathers
+    (np.float32, 137),  # hybrid; without it 32x64 LDS tiles
+    (np.float32, 161),  # hybrid; without it 32x32 LDS tiles
+    (np.float32, 181),  # hybrid; without it global gathers
     (np.float64, 73),   # 64x64
     (np.float64, 81),   # 32x64
 ])
-def test_replay_tile_shapes(dtype, klen):
+def test_replay_tile_shapes(dtype, klen, hybrid, monkeypatch):
+    monkeypatch.setenv("RLIC_REPLAY_HYBRID", hybrid)
     rng = np.random.default_rng(klen)
     tex, u, v = _fields(rng, (70, 130), dtype, "holes")
     k = rng.random(klen).astype(dtype)
@@ -304,7 +306,7 @@ def test_replay_tile_shapes(dtype, klen):
                            iterations=2)
         got = rlic_amd.convolve(tex, u, v, kernel=k, uv_mode="polarization", boundaries=b,
                                 iterations=2)
-        _assert_same(got, exp, f"{np.dtype(dtype).name} L={klen} {b}")
+        _assert_same(got, exp, f"{np.dtype(dtype).name} L={klen} {b} hybrid={hybrid}")
 
 
 @pytest.mark.parametrize("shape", [(33, 1), (14, 107), (65, 40), (63, 129), (130, 3), (128, 64)])
@@ -433,8 +435,8 @@ def test_lds_box_kernel_lengths(klen):
 
 
 @pytest.mark.parametrize("dtype, klen", [
-    (np.float32, 167), (np.float32, 200), (np.float32, 257),   # hybrid halo 64
-    (np.float64, 111), (np.float64, 150), (np.float64, 257),   # hybrid halo 32
+    (np.float32, 131), (np.float32, 167), (np.float32, 200), (np.float32, 257),  # halo 64
+    (np.float64, 111), (np.float64, 150), (np.float64, 257),                     # halo 32
 ])
 def test_hybrid_replay(dtype, klen, monkeypatch):
     """Kernel lengths whose whole LDS box does not fit replay with the
