@@ -177,10 +177,18 @@ const char* plan_kernel_name(const ReplayPlan& p) { return p.lds ? "lic_replay_l
 // L = 151, 165); else the largest fitting smaller box (in double the
 // hybrid's register spills make those faster); else the hybrid; else L2
 // gathers over the tracing pass's tile map.
+// The hybrid stages a 192 x 192 (float) box per 64 x 64 tile whatever the
+// image size: below 2^18 pixels (512 x 512) the staging outweighs the
+// gathers it saves (a loop over 260 fields of sides 32..256 took twice as
+// long), so smaller images keep the L2-gather replay.  RLIC_REPLAY_HYBRID:
+// 0 never, 1 (default) from 2^18 pixels, 2 at any size (tests).
+constexpr int64_t kHybridMinPixels = int64_t{1} << 18;
+
 template <typename T>
 bool hybrid_allowed(int32_t klen, int64_t n) {
-  return env_int("RLIC_REPLAY_LDS", 1) != 0 && env_int("RLIC_REPLAY_HYBRID", 1) != 0 &&
-         rlic::lds_hybrid_fits<T>(klen, n);
+  const int mode = env_int("RLIC_REPLAY_HYBRID", 1);
+  return env_int("RLIC_REPLAY_LDS", 1) != 0 && mode != 0 &&
+         (n >= kHybridMinPixels || mode == 2) && rlic::lds_hybrid_fits<T>(klen, n);
 }
 
 template <typename T>

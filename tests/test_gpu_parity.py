@@ -287,7 +287,7 @@ def test_replay_paths_match(lds, dtype, monkeypatch):
             _assert_same(got, exp, f"lds={lds} {shape} L={klen} {b}")
 
 
-@pytest.mark.parametrize("hybrid", ["1", "0"])
+@pytest.mark.parametrize("hybrid", ["2", "0"])
 @pytest.mark.parametrize("dtype,klen", [
     (np.float32, 65),   # 64x64 LDS tiles
     (np.float32, 137),  # hybrid; without it 32x64 LDS tiles
@@ -451,9 +451,41 @@ def test_hybrid_replay(dtype, klen, monkeypatch):
             tex, u, v = _fields(rng, shape, dtype, kind)
             for b in ("periodic", {"x": "periodic", "y": "closed"}):
                 exp = cpu.convolve(tex, u, v, kernel=k, boundaries=b, iterations=3)
-                monkeypatch.setenv("RLIC_REPLAY_HYBRID", "1")
+                monkeypatch.setenv("RLIC_REPLAY_HYBRID", "2")  # also below 2^18 pixels
                 got = rlic_amd.convolve(tex, u, v, kernel=k, boundaries=b, iterations=3)
                 _assert_same(got, exp, f"hybrid {shape} {kind} {b}")
                 monkeypatch.setenv("RLIC_REPLAY_HYBRID", "0")
                 l2 = rlic_amd.convolve(tex, u, v, kernel=k, boundaries=b, iterations=3)
                 _assert_same(l2, exp, f"l2 {shape} {kind} {b}")
+
+
+def test_hybrid_is_the_default_from_2_18_pixels(gpu_device, monkeypatch):
+    """Untuned, a float field of at least 2^18 pixels whose whole box does
+    not fit replays with the hybrid plan (a lic_replay_lds launch per pass);
+    it gives the oracle's bits and the L2-gather replay's."""
+    import torch
+
+    from rlic_amd import _native
+
+    rng = np.random.default_rng(201)
+    shape, klen = (520, 530), 201
+    tex, u, v = _fields(rng, shape, np.float32, "holes")
+    k = rng.random(klen).astype(np.float32)
+    t = [torch.from_numpy(a).to(gpu_device) for a in (tex, u, v, k)]
+    kw = dict(kernel=t[3], boundaries={"x": "periodic", "y": "closed"}, iterations=3)
+    monkeypatch.setenv("RLIC_AUTOTUNE", "0")
+    monkeypatch.setenv("RLIC_REPLAY_HYBRID", "1")
+    _native.profiler_enable(True)
+    _native.profiler_read()
+    try:
+        got = rlic_amd.convolve(t[0], t[1], t[2], **kw).cpu().numpy()
+        prof = _native.profiler_read()
+    finally:
+        _native.profiler_enable(False)
+    assert prof.get("lic_replay_lds", {}).get("count") == 2 and "lic_replay" not in prof
+    monkeypatch.setenv("RLIC_REPLAY_HYBRID", "0")
+    l2 = rlic_amd.convolve(t[0], t[1], t[2], **kw).cpu().numpy()
+    _assert_same(got, l2, "hybrid vs L2")
+    exp = cpu.convolve(tex, u, v, kernel=k, boundaries={"x": "periodic", "y": "closed"},
+                       iterations=3, threads=8)
+    _assert_same(got, exp, "hybrid vs oracle")
