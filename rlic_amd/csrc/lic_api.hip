@@ -171,17 +171,11 @@ hipError_t launch_plan(const ReplayPlan& p, const T* src, T* dst, const T* kerne
 
 const char* plan_kernel_name(const ReplayPlan& p) { return p.lds ? "lic_replay_lds" : "lic_replay"; }
 
-// Default plan: a 64x64 LDS tile when its box fits (RLIC_REPLAY_LDS=0:
-// never); else, in float, the hybrid when it fits (RLIC_REPLAY_HYBRID=0:
-// never; measured 1.5x faster per pass than the 32x64 and 32x32 boxes at
-// L = 151, 165); else the largest fitting smaller box (in double the
-// hybrid's register spills make those faster); else the hybrid; else L2
-// gathers over the tracing pass's tile map.
 // The hybrid stages a 192 x 192 (float) box per 64 x 64 tile whatever the
 // image size: below 2^18 pixels (512 x 512) the staging outweighs the
 // gathers it saves (a loop over 260 fields of sides 32..256 took twice as
-// long), so smaller images keep the L2-gather replay.  RLIC_REPLAY_HYBRID:
-// 0 never, 1 (default) from 2^18 pixels, 2 at any size (tests).
+// long), so smaller images do without it.  RLIC_REPLAY_HYBRID: 0 never,
+// 1 (default) from 2^18 pixels, 2 at any size (tests).
 constexpr int64_t kHybridMinPixels = int64_t{1} << 18;
 
 template <typename T>
@@ -191,22 +185,27 @@ bool hybrid_allowed(int32_t klen, int64_t n) {
          (n >= kHybridMinPixels || mode == 2) && rlic::lds_hybrid_fits<T>(klen, n);
 }
 
+// Default plan: the hybrid when it applies (kmid above its halo, an image
+// of at least 2^18 pixels; RLIC_REPLAY_HYBRID=0: never), else the largest
+// LDS tile whose whole box fits (RLIC_REPLAY_LDS=0: none), else L2 gathers
+// over the tracing pass's tile map.  Measured per pass (4096^2 float,
+// 2048^2 double): the hybrid beat every whole box, e.g. float L=135 0.59
+// vs 0.64 ms (64x64 box), L=151 0.76 vs 1.27 ms (32x32), double L=75 0.12
+// vs 0.15 ms (64x64): its box is smaller and only the last few steps of a
+// line leave it.
 template <typename T>
 ReplayPlan default_plan(int32_t klen, int64_t n) {
+  if (hybrid_allowed<T>(klen, n)) return {2, 64, 64, 0, 0};
   const rlic::LdsTile t = rlic::lds_replay_tile<T>(klen);
-  const bool lds = env_int("RLIC_REPLAY_LDS", 1) != 0;
-  if (lds && t.h == 64) return {1, 64, 64, 0, 0};
-  const bool hybrid = hybrid_allowed<T>(klen, n);
-  if (hybrid && sizeof(T) == 4) return {2, 64, 64, 0, 0};
-  if (lds && t.h > 0) return {1, t.h, t.w, 0, 0};
-  if (hybrid) return {2, 64, 64, 0, 0};
+  if (env_int("RLIC_REPLAY_LDS", 1) != 0 && t.h > 0) return {1, t.h, t.w, 0, 0};
   return {0, 0, 0, env_int("RLIC_TILE_BAND", 16), env_int("RLIC_WAVE_W_LOG2", 3)};
 }
 
 // Replay autotuner.  The first call with at least 3 passes on a field of at
 // least 2^18 pixels for a new (dtype, shape, kernel length, profile) times
 // its pass 2 under every candidate plan — each LDS tile whose box fits
-// (64x64, 32x64, 32x32) and L2 gathers with two strip heights — on the
+// (64x64, 32x64, 32x32), the hybrid when it applies, and L2 gathers with
+// two strip heights — on the
 // caller's stream with HIP events (all candidates write the same bits), and
 // keeps the fastest for that key.  RLIC_AUTOTUNE=0 disables it;
 // RLIC_AUTOTUNE_CACHE=<file> loads and appends the table as text.

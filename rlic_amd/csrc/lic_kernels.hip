@@ -10,6 +10,8 @@
 //               only gathers the new texture, in the same summation order
 //   lic_copy2d  strided 2-D copy (iterations == 0)
 //   lic_repack_texture  texture slot refresh when passes are re-traced
+#include <type_traits>
+
 #include "lic_device.h"
 #include "lic_launch.h"
 #include "lic_steps.h"
@@ -527,10 +529,9 @@ __global__ __launch_bounds__(kLdsThreads) void lic_replay_lds(const T* __restric
   // crossed in the box); closed edges never leave the image (their clamps
   // were recorded as "stay")
   const int32_t ch_box = HYBRID ? halo / kChunkSteps : INT32_MAX;
-#pragma unroll
-  for (int dir = 0; dir < 2; ++dir) {
-    const int32_t nsteps = dir == 0 ? n_fwd : n_bwd;
-    const int32_t c0 = dir == 0 ? 0 : chunks_of(n_fwd);
+  // one direction (fwd: std::true_type for forward) of every line of the lane
+  auto direction = [&](auto fwd, int32_t nsteps, int32_t c0) {
+    constexpr bool kForward = decltype(fwd)::value;
     lds_byte* at[E];
     bool alive[E];
 #pragma unroll
@@ -538,13 +539,8 @@ __global__ __launch_bounds__(kLdsThreads) void lic_replay_lds(const T* __restric
       at[e] = at0[e];
       alive[e] = inside[e];
     }
-    if (dir == 0) {
-      replay_lines<T, FMA, true, E>(LdsWalk<T>{}, acc, at, alive, kernel, kmid, nsteps,
-                                    pair_delta, codes, np, p, c0, 0, ch_box);
-    } else {
-      replay_lines<T, FMA, false, E>(LdsWalk<T>{}, acc, at, alive, kernel, kmid, nsteps,
-                                     pair_delta, codes, np, p, c0, 0, ch_box);
-    }
+    replay_lines<T, FMA, kForward, E>(LdsWalk<T>{}, acc, at, alive, kernel, kmid, nsteps,
+                                      pair_delta, codes, np, p, c0, 0, ch_box);
     if constexpr (HYBRID) {
       if (nsteps > halo) {  // uniform
         // built here, next to its use, so that its descriptor stays in
@@ -573,19 +569,16 @@ __global__ __launch_bounds__(kLdsThreads) void lic_replay_lds(const T* __restric
             alive2[k] = alive[e0 + k];
             p2[k] = p[e0 + k];
           }
-          if (dir == 0) {
-            replay_lines<T, FMA, true, E2>(image, acc2, pos2, alive2, kernel, kmid, nsteps,
-                                           pair_image, codes, np, p2, c0, ch_box, INT32_MAX);
-          } else {
-            replay_lines<T, FMA, false, E2>(image, acc2, pos2, alive2, kernel, kmid, nsteps,
-                                            pair_image, codes, np, p2, c0, ch_box, INT32_MAX);
-          }
+          replay_lines<T, FMA, kForward, E2>(image, acc2, pos2, alive2, kernel, kmid, nsteps,
+                                             pair_image, codes, np, p2, c0, ch_box, INT32_MAX);
 #pragma unroll
           for (int k = 0; k < E2; ++k) acc[e0 + k] = acc2[k];
         }
       }
     }
-  }
+  };
+  direction(std::true_type{}, n_fwd, 0);
+  direction(std::false_type{}, n_bwd, chunks_of(n_fwd));
 #pragma unroll
   for (int e = 0; e < E; ++e)
     if (inside[e]) out[p[e]] = acc[e];

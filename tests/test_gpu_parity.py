@@ -293,8 +293,8 @@ def test_replay_paths_match(lds, dtype, monkeypatch):
     (np.float32, 137),  # hybrid; without it 32x64 LDS tiles
     (np.float32, 161),  # hybrid; without it 32x32 LDS tiles
     (np.float32, 181),  # hybrid; without it global gathers
-    (np.float64, 73),   # 64x64
-    (np.float64, 81),   # 32x64
+    (np.float64, 73),   # hybrid; without it 64x64
+    (np.float64, 81),   # hybrid; without it 32x64
 ])
 def test_replay_tile_shapes(dtype, klen, hybrid, monkeypatch):
     monkeypatch.setenv("RLIC_REPLAY_HYBRID", hybrid)
