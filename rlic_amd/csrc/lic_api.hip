@@ -149,9 +149,9 @@ int env_int(const char* name, int dflt) {
 }
 
 // How a call's passes 2..n replay: lds = 1, LDS-staged with a th x tw tile;
-// lds = 2, the hybrid (64 x 64 tiles, the first steps of each direction
-// from an LDS box, the rest from the image); lds = 0, L2 gathers over a
-// tile map (strip height `band`, wave shape).
+// lds = 2, the hybrid (64 x 64 tiles, the first `band` steps of each
+// direction from an LDS box of that halo, the rest from the image); lds =
+// 0, L2 gathers over a tile map (strip height `band`, wave shape).
 struct ReplayPlan {
   int32_t lds, th, tw, band, wave_w_log2;
 };
@@ -161,7 +161,8 @@ hipError_t launch_plan(const ReplayPlan& p, const T* src, T* dst, const T* kerne
                        const rlic::FieldGeom& g, int profile, const uint4* codes,
                        hipStream_t stream) {
   if (p.lds == 2)
-    return rlic::launch_replay_lds_hybrid<T>(src, kernel, klen, g, profile, dst, codes, stream);
+    return rlic::launch_replay_lds_hybrid<T>(src, kernel, klen, g, p.band, profile, dst, codes,
+                                             stream);
   if (p.lds)
     return rlic::launch_replay_lds_tile<T>(src, kernel, klen, g, rlic::LdsTile{p.th, p.tw},
                                            profile, dst, codes, stream);
@@ -179,10 +180,21 @@ const char* plan_kernel_name(const ReplayPlan& p) { return p.lds ? "lic_replay_l
 constexpr int64_t kHybridMinPixels = int64_t{1} << 18;
 
 template <typename T>
-bool hybrid_allowed(int32_t klen, int64_t n) {
+bool hybrid_allowed(int32_t klen, int64_t n, int32_t halo) {
   const int mode = env_int("RLIC_REPLAY_HYBRID", 1);
   return env_int("RLIC_REPLAY_LDS", 1) != 0 && mode != 0 &&
-         (n >= kHybridMinPixels || mode == 2) && rlic::lds_hybrid_fits<T>(klen, n);
+         (n >= kHybridMinPixels || mode == 2) && rlic::lds_hybrid_fits<T>(klen, n, halo);
+}
+
+// The hybrid's halo: the largest whose box fits (64 in float, 32 in
+// double), when kmid exceeds it; 0 otherwise.  A 32-cell halo in float lost
+// to the whole 64x64 box at every kernel length measured (L = 67..129 at
+// 4096^2: 0.48 vs 0.40 ms per pass at L=67, 1.08 vs 0.48 at L=129) and to
+// the 64-cell halo beyond (L=199: 2.28 vs 1.51 ms).
+template <typename T>
+int32_t hybrid_halo(int32_t klen, int64_t n) {
+  const int32_t halo = rlic::hybrid_max_halo<T>();
+  return hybrid_allowed<T>(klen, n, halo) ? halo : 0;
 }
 
 // Default plan: the hybrid when it applies (kmid above its halo, an image
@@ -195,7 +207,7 @@ bool hybrid_allowed(int32_t klen, int64_t n) {
 // line leave it.
 template <typename T>
 ReplayPlan default_plan(int32_t klen, int64_t n) {
-  if (hybrid_allowed<T>(klen, n)) return {2, 64, 64, 0, 0};
+  if (const int32_t halo = hybrid_halo<T>(klen, n)) return {2, 64, 64, halo, 0};
   const rlic::LdsTile t = rlic::lds_replay_tile<T>(klen);
   if (env_int("RLIC_REPLAY_LDS", 1) != 0 && t.h > 0) return {1, t.h, t.w, 0, 0};
   return {0, 0, 0, env_int("RLIC_TILE_BAND", 16), env_int("RLIC_WAVE_W_LOG2", 3)};
@@ -252,7 +264,7 @@ void tuner_load_locked() {
 // A plan from the table is used only if it can run here.
 template <typename T>
 bool plan_usable(const ReplayPlan& p, int32_t klen, int64_t n) {
-  if (p.lds == 2) return p.th == 64 && p.tw == 64 && rlic::lds_hybrid_fits<T>(klen, n);
+  if (p.lds == 2) return p.th == 64 && p.tw == 64 && rlic::lds_hybrid_fits<T>(klen, n, p.band);
   if (p.lds) {
     const rlic::LdsTile big = rlic::lds_replay_tile<T>(klen);
     const bool shape = (p.th == 64 && p.tw == 64) || (p.th == 32 && (p.tw == 64 || p.tw == 32));
@@ -283,7 +295,7 @@ std::vector<ReplayPlan> replay_candidates(int32_t klen, int64_t n) {
     if (big.h == 64 || big.w == 64) c.push_back({1, 32, 64, 0, 0});
     c.push_back({1, 32, 32, 0, 0});
   }
-  if (hybrid_allowed<T>(klen, n)) c.push_back({2, 64, 64, 0, 0});
+  if (const int32_t halo = hybrid_halo<T>(klen, n)) c.push_back({2, 64, 64, halo, 0});
   c.push_back({0, 0, 0, 16, 3});
   c.push_back({0, 0, 0, 4, 3});
   return c;

@@ -220,13 +220,11 @@ inline size_t lds_replay_bytes(int32_t th, int32_t tw, int32_t halo, size_t elem
   return tables * kLdsPairTableBytes + lds_replay_dynamic_bytes(th, tw, halo, elem);
 }
 
-// Hybrid replay: a 64 x 64 tile whose box holds the first kHybridHalo steps
-// of each direction (a multiple of the 32-step chunk); later steps gather
-// from the image through buffer loads.
-template <typename T>
-constexpr int32_t hybrid_halo() {
-  return sizeof(T) == 4 ? 64 : 32;
-}
+// Hybrid replay: a 64 x 64 tile whose box holds the first `halo` steps of
+// each direction (32 or 64, a multiple of the 32-step chunk); later steps
+// gather from the image through buffer loads.  The largest halo whose box
+// fits (hybrid_max_halo): 64 in float (192 x 200 x 4 B), 32 in double
+// (128 x 136 x 8 B).
 
 __device__ __forceinline__ int32_t wrap_index(int32_t c, int32_t n) {
   c %= n;
@@ -412,15 +410,15 @@ __global__ __launch_bounds__(kLdsThreads) void lic_replay_lds(const T* __restric
                                                               const T* __restrict__ kernel,
                                                               int32_t klen, FieldGeom g,
                                                               TileMap m, T* __restrict__ out,
-                                                              const uint4* __restrict__ codes) {
+                                                              const uint4* __restrict__ codes,
+                                                              int32_t box_halo) {
   constexpr int E = TH * TW / kLdsThreads;  // pixels per lane: 1, 2 or 4
   constexpr int32_t elem = static_cast<int32_t>(sizeof(T));
   extern __shared__ __align__(16) unsigned char smem[];
   const int32_t kmid = klen / 2;
   // the box holds every step (halo kmid), or with HYBRID the first
-  // hybrid_halo() steps of each direction (the launcher picks HYBRID only
-  // when kmid is larger)
-  const int32_t halo = HYBRID ? hybrid_halo<T>() : kmid;
+  // box_halo steps of each direction (a multiple of 32 below kmid)
+  const int32_t halo = HYBRID ? box_halo : kmid;
   const int32_t bh = lds_box_h(TH, halo), bw = lds_box_w(TW, halo);
   const int32_t stride = lds_box_stride(TW, halo);
   __shared__ int2 pair_delta[256];
@@ -765,15 +763,16 @@ bool lds_replay_fits(int32_t klen) {
 }
 
 template <typename T>
-bool lds_hybrid_fits(int32_t klen, int64_t n) {
-  return klen / 2 > hybrid_halo<T>() && fits_buffer(n * static_cast<int64_t>(sizeof(T))) &&
-         lds_replay_bytes(64, 64, hybrid_halo<T>(), sizeof(T), 2) <=
+bool lds_hybrid_fits(int32_t klen, int64_t n, int32_t halo) {
+  return halo > 0 && halo % kChunkSteps == 0 && halo <= hybrid_max_halo<T>() &&
+         klen / 2 > halo && fits_buffer(n * static_cast<int64_t>(sizeof(T))) &&
+         lds_replay_bytes(64, 64, halo, sizeof(T), 2) <=
              static_cast<size_t>(lds_replay_limit_bytes());
 }
 
 template <typename T, bool FMA, int TH, int TW, bool HYBRID = false>
 static void replay_lds_one(const T* x, const T* kernel, int32_t klen, const FieldGeom& g, T* out,
-                           const uint4* codes, hipStream_t stream) {
+                           const uint4* codes, hipStream_t stream, int32_t hybrid_halo = 0) {
   TileMap m;
   m.wave_w_log2 = 3;
   m.tiles_x = (g.nx + TW - 1) / TW;
@@ -782,13 +781,13 @@ static void replay_lds_one(const T* x, const T* kernel, int32_t klen, const Fiel
   m.per_xcd = m.n_tiles / kXcds;
   m.rem_xcd = m.n_tiles % kXcds;
   m.band = 4;
-  const int32_t halo = HYBRID ? hybrid_halo<T>() : klen / 2;
+  const int32_t halo = HYBRID ? hybrid_halo : klen / 2;
   const size_t bytes = lds_replay_dynamic_bytes(TH, TW, halo, sizeof(T));
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&lic_replay_lds<T, FMA, TH, TW, HYBRID>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   (void)hipGetLastError();  // the attribute is advisory on this target
   hipLaunchKernelGGL((lic_replay_lds<T, FMA, TH, TW, HYBRID>), dim3(m.n_tiles), dim3(kLdsThreads),
-                     bytes, stream, x, kernel, klen, g, m, out, codes);
+                     bytes, stream, x, kernel, klen, g, m, out, codes, halo);
 }
 
 template <typename T, bool FMA>
@@ -845,12 +844,14 @@ hipError_t launch_replay_lds_tile(const T* x, const T* kernel, int32_t klen, con
 
 template <typename T>
 hipError_t launch_replay_lds_hybrid(const T* x, const T* kernel, int32_t klen, const FieldGeom& g,
-                                    int profile, T* out, const uint4* codes, hipStream_t stream) {
-  if (!lds_hybrid_fits<T>(klen, static_cast<int64_t>(g.ny) * g.nx)) return hipErrorInvalidValue;
+                                    int32_t halo, int profile, T* out, const uint4* codes,
+                                    hipStream_t stream) {
+  if (!lds_hybrid_fits<T>(klen, static_cast<int64_t>(g.ny) * g.nx, halo))
+    return hipErrorInvalidValue;
   if (profile & 1) {
-    replay_lds_one<T, true, 64, 64, true>(x, kernel, klen, g, out, codes, stream);
+    replay_lds_one<T, true, 64, 64, true>(x, kernel, klen, g, out, codes, stream, halo);
   } else {
-    replay_lds_one<T, false, 64, 64, true>(x, kernel, klen, g, out, codes, stream);
+    replay_lds_one<T, false, 64, 64, true>(x, kernel, klen, g, out, codes, stream, halo);
   }
   return hipGetLastError();
 }
@@ -873,10 +874,10 @@ hipError_t launch_replay_lds_hybrid(const T* x, const T* kernel, int32_t klen, c
                                            T*, const uint4*, hipStream_t);                      \
   template hipError_t launch_replay_lds_tile<T>(const T*, const T*, int32_t, const FieldGeom&,  \
                                                 LdsTile, int, T*, const uint4*, hipStream_t);  \
-  template bool lds_hybrid_fits<T>(int32_t, int64_t);                                          \
+  template bool lds_hybrid_fits<T>(int32_t, int64_t, int32_t);                                 \
   template hipError_t launch_replay_lds_hybrid<T>(const T*, const T*, int32_t,                 \
-                                                  const FieldGeom&, int, T*, const uint4*,     \
-                                                  hipStream_t);
+                                                  const FieldGeom&, int32_t, int, T*,          \
+                                                  const uint4*, hipStream_t);
 
 RLIC_INSTANTIATE(float)
 RLIC_INSTANTIATE(double)

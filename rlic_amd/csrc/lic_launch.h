@@ -92,15 +92,20 @@ hipError_t launch_replay_lds_tile(const T* x, const T* kernel, int32_t klen, con
                                   LdsTile tile, int profile, T* out, const uint4* codes,
                                   hipStream_t stream);
 
-// Hybrid replay for kernels whose full box does not fit: 64x64 tiles whose
-// box holds the first 64 (float) or 32 (double) steps of each direction,
-// later steps gathered from the image with buffer loads.  lds_hybrid_fits:
-// kmid exceeds that halo, the box fits and the image (n elements) is
+// Hybrid replay: 64x64 tiles whose box holds the first `halo` steps of each
+// direction (32 or 64; at most 32 in double), later steps gathered from
+// the image with buffer loads.  lds_hybrid_fits: the halo is one of those
+// and below kmid, the box fits and the image (n elements) is
 // buffer-addressable; otherwise the launch returns hipErrorInvalidValue.
 template <typename T>
-bool lds_hybrid_fits(int32_t klen, int64_t n);
+constexpr int32_t hybrid_max_halo() {
+  return sizeof(T) == 4 ? 64 : 32;
+}
+template <typename T>
+bool lds_hybrid_fits(int32_t klen, int64_t n, int32_t halo);
 template <typename T>
 hipError_t launch_replay_lds_hybrid(const T* x, const T* kernel, int32_t klen, const FieldGeom& g,
-                                    int profile, T* out, const uint4* codes, hipStream_t stream);
+                                    int32_t halo, int profile, T* out, const uint4* codes,
+                                    hipStream_t stream);
 
 }  // namespace rlic

@@ -41,7 +41,10 @@ def test_tuned_calls_give_the_same_bits(monkeypatch, tmp_path, klen, lds_fits):
     assert len(lines) == 1 and lines[0].startswith("rlic-replay-plan 4 520 530")
     # every candidate plan, loaded one at a time, gives the same bits
     monkeypatch.delenv("RLIC_AUTOTUNE_CACHE")
-    plans = (["1 64 64 0 0", "1 32 64 0 0", "1 32 32 0 0"] if lds_fits else ["2 64 64 0 0"]) + \
+    # hybrid plans carry the box halo in the band field: 64 (the default
+    # for float) and 32
+    plans = (["1 64 64 0 0", "1 32 64 0 0", "1 32 32 0 0"] if lds_fits
+             else ["2 64 64 64 0", "2 64 64 32 0"]) + \
         ["0 0 0 16 3", "0 0 0 4 3", "0 0 0 8 4"]
     for plan in plans:
         one = tmp_path / "one.txt"
@@ -57,7 +60,7 @@ def test_table_load_and_read_without_a_gpu(tmp_path):
     path = tmp_path / "plans.txt"
     path.write_text("rlic-replay-plan 8 100 200 33 3 1 32 64 0 0 0.5\n"
                     "rlic-replay-plan 4 64 64 9 1 0 0 0 16 3 0.25\n"
-                    "rlic-replay-plan 4 80 90 257 3 2 64 64 0 0 0.75\n")
+                    "rlic-replay-plan 4 80 90 257 3 2 64 64 64 0 0.75\n")
     _native.autotune_clear()
     assert _native.autotune_load(path) == 3
     table = _native.autotune_table()
