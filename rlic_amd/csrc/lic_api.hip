@@ -172,18 +172,26 @@ hipError_t launch_plan(const ReplayPlan& p, const T* src, T* dst, const T* kerne
 
 const char* plan_kernel_name(const ReplayPlan& p) { return p.lds ? "lic_replay_lds" : "lic_replay"; }
 
-// The hybrid stages a 192 x 192 (float) box per 64 x 64 tile whatever the
-// image size: below 2^18 pixels (512 x 512) the staging outweighs the
-// gathers it saves (a loop over 260 fields of sides 32..256 took twice as
-// long), so smaller images do without it.  RLIC_REPLAY_HYBRID: 0 never,
+// LDS plans stage a box per tile (192 x 192 for a 64 x 64 float tile at
+// kmid 64) whatever the image size.  Below 2^18 pixels (512 x 512) the
+// staging outweighs the gathers it saves: per-call loops over fields of
+// 2^12..2^18 pixels ran 1.1-1.5x faster with L2 gathers (8.7 vs 13.4 ms,
+// 12.9 vs 17.1 ms, 13.9 vs 15.2 ms by size class), and the hybrid doubled
+// a small-field loop, so smaller images use the L2-gather replay.
+// RLIC_REPLAY_LDS (whole boxes and hybrid) and RLIC_REPLAY_HYBRID: 0 never,
 // 1 (default) from 2^18 pixels, 2 at any size (tests).
-constexpr int64_t kHybridMinPixels = int64_t{1} << 18;
+constexpr int64_t kLdsMinPixels = int64_t{1} << 18;
+
+bool lds_allowed(int64_t n) {
+  const int mode = env_int("RLIC_REPLAY_LDS", 1);
+  return mode != 0 && (n >= kLdsMinPixels || mode == 2);
+}
 
 template <typename T>
 bool hybrid_allowed(int32_t klen, int64_t n, int32_t halo) {
   const int mode = env_int("RLIC_REPLAY_HYBRID", 1);
-  return env_int("RLIC_REPLAY_LDS", 1) != 0 && mode != 0 &&
-         (n >= kHybridMinPixels || mode == 2) && rlic::lds_hybrid_fits<T>(klen, n, halo);
+  return mode != 0 && (lds_allowed(n) || (mode == 2 && env_int("RLIC_REPLAY_LDS", 1) != 0)) &&
+         rlic::lds_hybrid_fits<T>(klen, n, halo);
 }
 
 // The hybrid's halo: the largest whose box fits (64 in float, 32 in
@@ -198,9 +206,8 @@ int32_t hybrid_halo(int32_t klen, int64_t n) {
 }
 
 // Default plan: the hybrid when it applies (kmid above its halo, an image
-// of at least 2^18 pixels; RLIC_REPLAY_HYBRID=0: never), else the largest
-// LDS tile whose whole box fits (RLIC_REPLAY_LDS=0: none), else L2 gathers
-// over the tracing pass's tile map.  Measured per pass (4096^2 float,
+// of at least 2^18 pixels), else the largest LDS tile whose whole box fits
+// (from 2^18 pixels), else L2 gathers over the tracing pass's tile map.  Measured per pass (4096^2 float,
 // 2048^2 double): the hybrid beat every whole box, e.g. float L=135 0.59
 // vs 0.64 ms (64x64 box), L=151 0.76 vs 1.27 ms (32x32), double L=75 0.12
 // vs 0.15 ms (64x64): its box is smaller and only the last few steps of a
@@ -209,7 +216,7 @@ template <typename T>
 ReplayPlan default_plan(int32_t klen, int64_t n) {
   if (const int32_t halo = hybrid_halo<T>(klen, n)) return {2, 64, 64, halo, 0};
   const rlic::LdsTile t = rlic::lds_replay_tile<T>(klen);
-  if (env_int("RLIC_REPLAY_LDS", 1) != 0 && t.h > 0) return {1, t.h, t.w, 0, 0};
+  if (lds_allowed(n) && t.h > 0) return {1, t.h, t.w, 0, 0};
   return {0, 0, 0, env_int("RLIC_TILE_BAND", 16), env_int("RLIC_WAVE_W_LOG2", 3)};
 }
 

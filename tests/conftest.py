@@ -1,5 +1,6 @@
 from __future__ import annotations
 
+import os
 import sys
 from pathlib import Path
 
@@ -8,6 +9,11 @@ import pytest
 ROOT = Path(__file__).resolve().parents[1]
 if str(ROOT) not in sys.path:
     sys.path.insert(0, str(ROOT))
+
+# The library replays images below 2^18 pixels with L2 gathers by default;
+# the test fields are small, so the session lets the LDS-box and hybrid
+# replays run at any size (tests of the defaults set these back to 1).
+os.environ.setdefault("RLIC_REPLAY_LDS", "2")
 
 
 def pytest_configure(config: pytest.Config) -> None:

@@ -270,7 +270,7 @@ def test_library_and_torch_share_one_hip_runtime():
     assert proc.returncode == 0 and proc.stdout.strip().endswith("ok"), proc.stdout + proc.stderr
 
 
-@pytest.mark.parametrize("lds", ["0", "1"])
+@pytest.mark.parametrize("lds", ["0", "2"])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_replay_paths_match(lds, dtype, monkeypatch):
     """Replay from LDS-staged tiles and from global gathers give the same bits,
@@ -474,6 +474,7 @@ def test_hybrid_is_the_default_from_2_18_pixels(gpu_device, monkeypatch):
     t = [torch.from_numpy(a).to(gpu_device) for a in (tex, u, v, k)]
     kw = dict(kernel=t[3], boundaries={"x": "periodic", "y": "closed"}, iterations=3)
     monkeypatch.setenv("RLIC_AUTOTUNE", "0")
+    monkeypatch.setenv("RLIC_REPLAY_LDS", "1")
     monkeypatch.setenv("RLIC_REPLAY_HYBRID", "1")
     _native.profiler_enable(True)
     _native.profiler_read()
@@ -489,3 +490,35 @@ def test_hybrid_is_the_default_from_2_18_pixels(gpu_device, monkeypatch):
     exp = cpu.convolve(tex, u, v, kernel=k, boundaries={"x": "periodic", "y": "closed"},
                        iterations=3, threads=8)
     _assert_same(got, exp, "hybrid vs oracle")
+
+
+@pytest.mark.parametrize("shape, kernel_name", [
+    ((300, 400), "lic_replay"),        # below 2^18 pixels: L2 gathers
+    ((520, 530), "lic_replay_lds"),    # from 2^18 pixels: the 64x64 LDS box
+])
+def test_default_replay_plan_by_image_size(shape, kernel_name, gpu_device, monkeypatch):
+    """With the default switches, images below 2^18 pixels replay with L2
+    gathers and larger ones from LDS boxes; both give the oracle's bits."""
+    import torch
+
+    from rlic_amd import _native
+
+    rng = np.random.default_rng(shape[0])
+    tex, u, v = _fields(rng, shape, np.float32, "random")
+    k = rng.random(65).astype(np.float32)
+    t = [torch.from_numpy(a).to(gpu_device) for a in (tex, u, v, k)]
+    monkeypatch.setenv("RLIC_AUTOTUNE", "0")
+    monkeypatch.setenv("RLIC_REPLAY_LDS", "1")
+    monkeypatch.setenv("RLIC_REPLAY_HYBRID", "1")
+    _native.profiler_enable(True)
+    _native.profiler_read()
+    try:
+        got = rlic_amd.convolve(t[0], t[1], t[2], kernel=t[3], boundaries="periodic",
+                                iterations=2).cpu().numpy()
+        prof = _native.profiler_read()
+    finally:
+        _native.profiler_enable(False)
+    replays = {name for name in prof if name.startswith("lic_replay")}
+    assert replays == {kernel_name}
+    exp = cpu.convolve(tex, u, v, kernel=k, boundaries="periodic", iterations=2, threads=8)
+    _assert_same(got, exp, f"{shape}")
