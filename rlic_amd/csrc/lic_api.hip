@@ -816,6 +816,23 @@ int server_read(rlic::Server* server, int64_t handle, T* out) {
 
 }  // namespace
 
+namespace rlic {
+
+template <typename T>
+hipError_t replay_pass(const T* src, T* dst, const T* kernel, int32_t klen, const FieldGeom& g,
+                       int profile, const uint4* codes, hipStream_t stream) {
+  const ReplayPlan plan = default_plan<T>(klen, static_cast<int64_t>(g.ny) * g.nx);
+  LaunchTimer timer(plan_kernel_name(plan), stream);
+  return launch_plan<T>(plan, src, dst, kernel, klen, g, profile, codes, stream);
+}
+
+template hipError_t replay_pass<float>(const float*, float*, const float*, int32_t,
+                                       const FieldGeom&, int, const uint4*, hipStream_t);
+template hipError_t replay_pass<double>(const double*, double*, const double*, int32_t,
+                                        const FieldGeom&, int, const uint4*, hipStream_t);
+
+}  // namespace rlic
+
 namespace {
 
 template <typename T>

@@ -14,8 +14,18 @@
 
 namespace rlic {
 
+// One replay pass (src -> dst along the move record `codes`) of a field as
+// a single-field launch with the default replay plan (LDS box, hybrid or L2
+// gathers; no tuning).  `kernel`: the first weight of a copy padded with
+// kernel_pad() zeros on each side.  Defined in lic_api.hip.
+template <typename T>
+hipError_t replay_pass(const T* src, T* dst, const T* kernel, int32_t klen, const FieldGeom& g,
+                       int profile, const uint4* codes, hipStream_t stream);
+
 // Device state of one admitted field, in one allocation:
-// [weights | cells | buffer 0 | buffer 1 | move record].
+// [padded weights | cells | buffer 0 | buffer 1 | move record].  The weights
+// are kept with kernel_pad() zeros on each side (the single-field replay
+// kernels read them so); `weights` points at the first weight.
 template <typename T>
 struct Resident {
   FieldGeom g{};

@@ -5,7 +5,9 @@
 // that stays resident until the field is released; each step then runs one
 // pass of every field that has passes left, all fields of a dtype in one
 // persistent launch over square regions (launch_queue without waits: the
-// step boundary orders the passes).  A field's first pass traces and, when
+// step boundary orders the passes), except the replay passes of fields of
+// at least 2^18 pixels, which run as single-field launches (LDS box or
+// hybrid replay, as a single convolve call would).  A field's first pass traces and, when
 // it keeps a move record, records the moves; later passes (and follow-up
 // passes requested after the field finished) replay them.  The two output
 // buffers alternate, so a field's passes give the same bits as one
@@ -36,10 +38,13 @@ Server::~Server() {
   (void)hipStreamSynchronize(stream_);
 }
 
+// Replay passes of fields this large run as single-field launches.
+constexpr int64_t kSingleReplayMinPixels = int64_t{1} << 18;
+
 template <typename T>
 size_t Server::field_bytes(int64_t ny, int64_t nx, int64_t klen, bool record) {
   const size_t n = static_cast<size_t>(ny * nx);
-  size_t b = align256(sizeof(T) * static_cast<size_t>(std::max<int64_t>(klen, 1)));
+  size_t b = align256(sizeof(T) * static_cast<size_t>(std::max<int64_t>(klen, 0) + 2 * kernel_pad()));
   b += align256(sizeof(Cell<T>) * n);
   b += 2 * align256(sizeof(T) * n);
   if (record) b += align256(sizeof(uint4) * static_cast<size_t>(code_chunks(static_cast<int32_t>(klen))) * n);
@@ -70,8 +75,10 @@ int Server::admit(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
   }
   const size_t n = static_cast<size_t>(ny * nx);
   char* p = static_cast<char*>(r.block);
-  r.weights = reinterpret_cast<T*>(p);
-  p += align256(sizeof(T) * static_cast<size_t>(std::max<int64_t>(klen, 1)));
+  const int32_t pad = kernel_pad();
+  T* padded = reinterpret_cast<T*>(p);
+  r.weights = padded + pad;
+  p += align256(sizeof(T) * static_cast<size_t>(std::max<int64_t>(klen, 0) + 2 * pad));
   r.cells = reinterpret_cast<Cell<T>*>(p);
   p += align256(sizeof(Cell<T>) * n);
   r.buf[0] = reinterpret_cast<T*>(p);
@@ -80,8 +87,13 @@ int Server::admit(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
   p += align256(sizeof(T) * n);
   r.codes = record ? reinterpret_cast<uint4*>(p) : nullptr;
   r.target = iterations;
-  hipError_t e = hipMemcpyAsync(r.weights, kernel, sizeof(T) * static_cast<size_t>(klen),
-                                hipMemcpyDefault, stream_);
+  // zeros on both sides of the weights (no graph captures this stream)
+  hipError_t e = hipMemsetAsync(padded, 0, sizeof(T) * pad, stream_);
+  if (e == hipSuccess)
+    e = hipMemsetAsync(r.weights + klen, 0, sizeof(T) * pad, stream_);
+  if (e == hipSuccess && klen > 0)
+    e = hipMemcpyAsync(r.weights, kernel, sizeof(T) * static_cast<size_t>(klen),
+                       hipMemcpyDefault, stream_);
   if (e == hipSuccess && n > 0)
     e = launch_pack<T>(tex, tsy, tsx, u, usy, usx, v, vsy, vsx, static_cast<int32_t>(ny),
                        static_cast<int32_t>(nx), r.cells, stream_);
@@ -101,14 +113,20 @@ int Server::step_dtype(std::vector<int64_t>& finished, std::string& err) {
   std::vector<BatchField<T>> fields;
   std::vector<const T*> ksrc;
   std::vector<Segment> segs;
-  std::vector<Resident<T>*> stepped;
-  std::vector<int64_t> handles;
+  std::vector<Resident<T>*> stepped, singles;
+  std::vector<int64_t> handles, single_handles;
   for (auto& kv : table<T>()) {
     Resident<T>& r = kv.second;
     if (r.done >= r.target) continue;
     if (static_cast<int64_t>(r.g.ny) * r.g.nx == 0) {  // nothing to compute
       r.done = r.target;
       finished.push_back(kv.first);
+      continue;
+    }
+    if (r.done > 0 && r.recorded &&
+        static_cast<int64_t>(r.g.ny) * r.g.nx >= kSingleReplayMinPixels) {
+      singles.push_back(&r);
+      single_handles.push_back(kv.first);
       continue;
     }
     BatchField<T> f;
@@ -139,16 +157,39 @@ int Server::step_dtype(std::vector<int64_t>& finished, std::string& err) {
     stepped.push_back(&r);
     handles.push_back(kv.first);
   }
-  if (fields.empty()) return RLIC_OK;
-  const int rc = launch_queue<T>(fields, ksrc, segs, /*deps=*/false, /*pack=*/false, profile_,
-                                 stream_, err);
-  if (rc != RLIC_OK) return rc;
+  for (Resident<T>* r : singles) {
+    const int out_buf = 1 - r->cur;
+    if (hipError_t e = replay_pass<T>(r->buf[r->cur], r->buf[out_buf], r->weights, r->klen,
+                                      r->g, profile_, r->codes, stream_);
+        e != hipSuccess) {
+      (void)hipGetLastError();
+      err = std::string("replay pass: ") + hipGetErrorString(e);
+      return RLIC_ERR_HIP;
+    }
+  }
+  if (!fields.empty()) {
+    const int rc = launch_queue<T>(fields, ksrc, segs, /*deps=*/false, /*pack=*/false, profile_,
+                                   stream_, err);
+    if (rc != RLIC_OK) return rc;
+  } else if (!singles.empty()) {
+    if (hipError_t e = hipStreamSynchronize(stream_); e != hipSuccess) {  // the step is synchronous
+      (void)hipGetLastError();
+      err = std::string("serving step: ") + hipGetErrorString(e);
+      return RLIC_ERR_HIP;
+    }
+  }
   for (size_t k = 0; k < stepped.size(); ++k) {
     Resident<T>& r = *stepped[k];
     if (r.done == 0 && r.codes != nullptr) r.recorded = true;
     r.cur = r.cur < 0 ? 0 : 1 - r.cur;
     ++r.done;
     if (r.done == r.target) finished.push_back(handles[k]);
+  }
+  for (size_t k = 0; k < singles.size(); ++k) {
+    Resident<T>& r = *singles[k];
+    r.cur = 1 - r.cur;
+    ++r.done;
+    if (r.done == r.target) finished.push_back(single_handles[k]);
   }
   return RLIC_OK;
 }

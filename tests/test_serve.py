@@ -205,3 +205,41 @@ def test_background_loop_with_concurrent_submitters(gpu_device):
         if (k, "f") in results:
             exp = _expected(r, r["iterations"] + 2)
             assert np.array_equal(results[(k, "f")].result, exp, equal_nan=True), k
+
+
+@pytest.mark.gpu
+def test_large_fields_replay_as_single_launches(gpu_device):
+    """Replay passes of fields of at least 2^18 pixels run as single-field
+    launches (LDS box or hybrid replay) inside the serving step, next to the
+    persistent launch of the small fields; every request, follow-ups
+    included, gives the bits of one convolve call."""
+    from rlic_amd import _native
+    from rlic_amd.serve import Server
+
+    rng = np.random.default_rng(13)
+    reqs = _requests(rng, 4)
+    for k, (dtype, klen) in enumerate([(np.float32, 65), (np.float32, 201), (np.float64, 99)]):
+        shape = (520, 530)
+        u = (rng.random(shape) - 0.5).astype(dtype)
+        u[rng.random(shape) < 0.02] = np.nan
+        reqs.append(dict(
+            texture=rng.random(shape).astype(dtype), u=u,
+            v=(rng.random(shape) - 0.5).astype(dtype), kernel=rng.random(klen).astype(dtype),
+            uv_mode=("velocity", "polarization")[k % 2],
+            boundaries={"x": "periodic", "y": ("closed", "periodic")[k % 2]}, iterations=3))
+    _native.profiler_enable(True)
+    _native.profiler_read()
+    try:
+        with Server(1 << 32, device=gpu_device) as srv:
+            handles = [_submit(srv, r, keep=True) for r in reqs]
+            srv.run()
+            more = [srv.follow_up(h, 2) for h in handles[-3:]]
+            srv.run()
+        prof = _native.profiler_read()
+    finally:
+        _native.profiler_enable(False)
+    assert prof.get("lic_replay_lds", {}).get("count", 0) >= 3 * 4  # 3 fields x (2 + 2) passes
+    for k, (r, h) in enumerate(zip(reqs, handles)):
+        assert np.array_equal(h.result, _expected(r), equal_nan=True), k
+    for r, h in zip(reqs[-3:], more):
+        assert np.array_equal(h.result, _expected(r, 5), equal_nan=True)
