@@ -37,12 +37,26 @@ struct __align__(32) Cell<double> {
 template <typename T>
 struct CellVal {
   T u, v, x;
+  T pad;  // a guarded array's jump bits (see pad_bits)
 };
+
+// The byte jump a guarded cell array keeps in a cell's pad slot (low 32
+// bits of its bit pattern; 0 in image cells).
+__device__ __forceinline__ uint32_t pad_bits(float pad) { return __builtin_bit_cast(uint32_t, pad); }
+__device__ __forceinline__ uint32_t pad_bits(double pad) {
+  return static_cast<uint32_t>(__builtin_bit_cast(uint64_t, pad));
+}
+__host__ __device__ inline float jump_pad(int32_t jump, float*) {
+  return __builtin_bit_cast(float, jump);
+}
+__host__ __device__ inline double jump_pad(int32_t jump, double*) {
+  return __builtin_bit_cast(double, static_cast<uint64_t>(static_cast<uint32_t>(jump)));
+}
 
 __device__ __forceinline__ CellVal<float> load_cell(const Cell<float>* __restrict__ p,
                                                     uint32_t idx) {
   const float4 c = *reinterpret_cast<const float4*>(p + idx);
-  return {c.x, c.y, c.z};
+  return {c.x, c.y, c.z, c.w};
 }
 
 __device__ __forceinline__ CellVal<double> load_cell(const Cell<double>* __restrict__ p,
@@ -50,7 +64,7 @@ __device__ __forceinline__ CellVal<double> load_cell(const Cell<double>* __restr
   const double2* q = reinterpret_cast<const double2*>(p + idx);
   const double2 a = q[0];
   const double2 b = q[1];
-  return {a.x, a.y, b.x};
+  return {a.x, a.y, b.x, b.y};
 }
 
 // ---------------------------------------------------------------------------
@@ -80,14 +94,15 @@ __device__ __forceinline__ double buf_load(const __amdgpu_buffer_rsrc_t r, uint3
 __device__ __forceinline__ CellVal<float> buf_load_cell(const __amdgpu_buffer_rsrc_t r,
                                                         uint32_t off, float*) {
   const float4 c = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
-  return {c.x, c.y, c.z};
+  return {c.x, c.y, c.z, c.w};
 }
 __device__ __forceinline__ CellVal<double> buf_load_cell(const __amdgpu_buffer_rsrc_t r,
                                                          uint32_t off, double*) {
   const double2 uv =
       __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
-  const double x = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off + 16u, 0, 0));
-  return {uv.x, uv.y, x};
+  const double2 xp =
+      __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off + 16u, 0, 0));
+  return {uv.x, uv.y, xp.x, xp.y};
 }
 
 // Cell gather at `pos`: a byte offset (BUF) or an element index (global).
@@ -224,6 +239,7 @@ struct FieldGeom {
 struct StepGeom {
   uint32_t nx, nxm1, last_row;  // row step, last column, last row offset
   bool x_lo_per, x_hi_per, y_lo_per, y_hi_per;  // periodic sides
+  uint32_t grow;  // row step of a guarded cell array ((ny + 2) x (nx + 2))
 };
 
 // Move codes recorded by the tracing pass (4 bits per step, 8 per word), a
@@ -269,6 +285,7 @@ __host__ __device__ inline StepGeom make_step_geom(const FieldGeom& g, uint32_t 
   s.x_hi_per = g.x_right != 0;
   s.y_lo_per = g.y_left != 0;
   s.y_hi_per = g.y_right != 0;
+  s.grow = static_cast<uint32_t>(g.nx + 2) * unit;
   return s;
 }
 

@@ -18,7 +18,15 @@ int32_t code_chunks(int32_t klen);
 template <typename T>
 hipError_t launch_pack(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t usy,
                        int64_t usx, const T* v, int64_t vsy, int64_t vsx, int32_t ny,
-                       int32_t nx, Cell<T>* cells, hipStream_t stream);
+                       int32_t nx, Cell<T>* cells, hipStream_t stream,
+                       const FieldGeom* guarded = nullptr);
+
+// The single-field trace's cell layout: guarded ((ny + 2) x (nx + 2) cells,
+// a ring of guard cells around the image, launch_pack given the field's
+// geometry)
+// when that array is buffer-addressable (< 4 GiB), else plain (ny x nx).
+template <typename T>
+bool trace_guarded(int32_t ny, int32_t nx);
 
 // Zeros on each side of the padded kernel-weight copy the streamline
 // kernels read (they load taps in groups of 8 without bounds checks).
@@ -29,7 +37,8 @@ template <typename T>
 hipError_t launch_pad_kernel(const T* kernel, int32_t klen, T* dst, hipStream_t stream);
 
 template <typename T>
-hipError_t launch_repack_texture(const T* x, int32_t n, Cell<T>* cells, hipStream_t stream);
+hipError_t launch_repack_texture(const T* x, const FieldGeom& g, bool guarded, Cell<T>* cells,
+                                 hipStream_t stream);
 
 template <typename T>
 hipError_t launch_copy2d(const T* src, int64_t sy, int64_t sx, int32_t ny, int32_t nx, T* dst,
