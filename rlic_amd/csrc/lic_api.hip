@@ -479,11 +479,9 @@ int run_on_device(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
     replay = code_bytes <= free_b / 2;
   }
   // one scratch block: padded weights | cells | ping-pong buffer | move record
-  // (the cells guarded, with a ring of guard cells, when buffer-addressable)
+  // (the cells a guarded array: a ring of guard cells around the image)
   auto up = [](size_t x) { return (x + 255) & ~size_t{255}; };
-  const bool guarded = rlic::trace_guarded<T>(ny, nx);
-  const size_t ncells = guarded ? static_cast<size_t>(ny + 2) * static_cast<size_t>(nx + 2)
-                                : static_cast<size_t>(n);
+  const size_t ncells = static_cast<size_t>(rlic::guarded_cells(ny, nx));
   const size_t off_cells = up(sizeof(T) * static_cast<size_t>(klen + 2 * pad));
   const size_t off_tmp = off_cells + up(sizeof(rlic::Cell<T>) * ncells);
   const size_t off_codes = off_tmp + (need_tmp ? up(sizeof(T) * static_cast<size_t>(n)) : 0);
@@ -508,8 +506,7 @@ int run_on_device(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
   {
     LaunchTimer timer("lic_pack", stream);
     RLIC_HIP_TRY(
-        rlic::launch_pack<T>(tex, tsy, tsx, u, usy, usx, v, vsy, vsx, ny, nx, c, stream,
-                             guarded ? &g : nullptr),
+        rlic::launch_pack<T>(tex, tsy, tsx, u, usy, usx, v, vsy, vsx, g, c, stream),
         "pack");
   }
   const rlic::TileMap m = rlic::make_tile_map(ny, nx, env_int("RLIC_TILE_BAND", 16),
@@ -592,7 +589,7 @@ int run_on_device(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
     }
     if (it != iterations) {
       LaunchTimer timer("lic_repack_texture", stream);
-      RLIC_HIP_TRY(rlic::launch_repack_texture<T>(tmp, g, guarded, c, stream),
+      RLIC_HIP_TRY(rlic::launch_repack_texture<T>(tmp, g, c, stream),
                    "repack");
     }
   }

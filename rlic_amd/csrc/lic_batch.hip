@@ -48,9 +48,10 @@ __global__ __launch_bounds__(256) void lic_pad_batch(const BatchField<T>* fields
   }
 }
 
-// Pack every field's (texture, u, v) into cells; a field with zero passes
-// gets its texture copied to `out` instead.  Block b covers 64 x 4 pixels of
-// the field whose pack_block0 range contains b.
+// Pack every field's (texture, u, v) into its guarded cell array (ring of
+// guard cells, see guarded_cell); a field with zero passes gets its texture
+// copied to `out` instead.  Block b covers 64 x 4 cells of the guarded
+// array of the field whose pack_block0 range contains b.
 template <typename T>
 __global__ __launch_bounds__(256) void lic_pack_batch(const BatchField<T>* fields, int32_t nfields) {
   const int64_t b = blockIdx.x;
@@ -61,22 +62,18 @@ __global__ __launch_bounds__(256) void lic_pack_batch(const BatchField<T>* field
   }
   const BatchField<T>& f = fields[lo];
   const int64_t local = b - f.pack_block0;
-  const int32_t bx = (f.g.nx + 63) / 64;
-  const int32_t j = static_cast<int32_t>(local % bx) * 64 + (threadIdx.x & 63);
-  const int32_t i = static_cast<int32_t>(local / bx) * 4 + (threadIdx.x >> 6);
-  if (i >= f.g.ny || j >= f.g.nx) return;
-  const T x = f.tex[i * f.tsy + j * f.tsx];
-  const int64_t idx = static_cast<int64_t>(i) * f.g.nx + j;
+  const int32_t bx = (f.g.nx + 2 + 63) / 64;
+  const int32_t gj = static_cast<int32_t>(local % bx) * 64 + (threadIdx.x & 63);
+  const int32_t gi = static_cast<int32_t>(local / bx) * 4 + (threadIdx.x >> 6);
+  if (gi >= f.g.ny + 2 || gj >= f.g.nx + 2) return;
   if (f.iterations == 0) {
-    f.out[idx] = x;
+    const int32_t i = gi - 1, j = gj - 1;
+    if (i >= 0 && i < f.g.ny && j >= 0 && j < f.g.nx)
+      f.out[static_cast<int64_t>(i) * f.g.nx + j] = f.tex[i * f.tsy + j * f.tsx];
     return;
   }
-  Cell<T> c;
-  c.u = f.u[i * f.usy + j * f.usx];
-  c.v = f.v[i * f.vsy + j * f.vsx];
-  c.x = x;
-  c.pad = T(0);
-  f.cells[idx] = c;
+  f.cells[static_cast<int64_t>(gi) * (f.g.nx + 2) + gj] =
+      guarded_cell(f.tex, f.tsy, f.tsx, f.u, f.usy, f.usx, f.v, f.vsy, f.vsx, f.g, gi, gj);
 }
 
 __global__ void lic_zero_u32(uint32_t* p, int64_t n) {
@@ -87,20 +84,20 @@ template <typename T, bool FMA, bool BRANCHLESS, int POL, int REC, bool BUF>
 __device__ __forceinline__ void batch_trace_pixel(const BatchField<T>& f,
                                                   const T* __restrict__ kernel, T* dst,
                                                   int32_t i0, int32_t j0) {
-  const CellGather<T, BUF> cell(f.cells, static_cast<int64_t>(f.g.ny) * f.g.nx);
+  const CellGather<T, BUF> cell(f.cells, guarded_cells(f.g.ny, f.g.nx));
   const StepGeom sg = make_step_geom(f.g, cell.unit);
   const int32_t kmid = f.klen / 2;
   const int32_t n_fwd = f.klen - 1 - kmid, n_bwd = kmid;
   const int64_t np = static_cast<int64_t>(f.g.ny) * f.g.nx;
   const int64_t p = static_cast<int64_t>(i0) * f.g.nx + j0;
-  const uint32_t row0 = static_cast<uint32_t>(i0) * sg.nx;
-  const uint32_t jj0 = static_cast<uint32_t>(j0) * cell.unit;
-  const CellVal<T> c0 = cell(row0 + jj0);
+  const uint32_t row0 =
+      static_cast<uint32_t>(i0 + 1) * sg.grow + static_cast<uint32_t>(j0 + 1) * cell.unit;
+  const CellVal<T> c0 = cell(row0);
   T acc = macc<FMA>(kernel[kmid], c0.x, T(0));  // src/lib.rs:375-383
   acc = trace_direction<T, FMA, BRANCHLESS, POL, true, REC, BUF>(
-      acc, c0.u, c0.v, row0, jj0, cell, kernel, kmid, n_fwd, sg, f.codes, np, p, 0);
+      acc, c0.u, c0.v, row0, cell, kernel, kmid, n_fwd, sg, f.codes, np, p, 0);
   acc = trace_direction<T, FMA, BRANCHLESS, POL, false, REC, BUF>(
-      acc, c0.u, c0.v, row0, jj0, cell, kernel, kmid, n_bwd, sg, f.codes, np, p,
+      acc, c0.u, c0.v, row0, cell, kernel, kmid, n_bwd, sg, f.codes, np, p,
       chunks_of(n_fwd));
   dst[p] = acc;
 }
@@ -364,11 +361,11 @@ int launch_queue(std::vector<BatchField<T>>& fields, const std::vector<const T*>
     f.regions_x = (f.g.nx + region - 1) / region;
     f.regions = f.regions_x * ((f.g.ny + region - 1) / region);
     f.pack_block0 = pack_blocks;
-    if (pack) pack_blocks += static_cast<int64_t>((f.g.nx + 63) / 64) * ((f.g.ny + 3) / 4);
+    if (pack) pack_blocks += static_cast<int64_t>((f.g.nx + 2 + 63) / 64) * ((f.g.ny + 2 + 3) / 4);
     f.kernel_off = pool_len + kKernelPad;
     pool_len += f.klen + 2 * kKernelPad;
-    const int64_t n = static_cast<int64_t>(f.g.ny) * f.g.nx;
-    if (n * static_cast<int64_t>(sizeof(Cell<T>)) > 0xFFFFFFFFll) buf = false;
+    if (guarded_cells(f.g.ny, f.g.nx) * static_cast<int64_t>(sizeof(Cell<T>)) > 0xFFFFFFFFll)
+      buf = false;
   }
   int64_t items = 0;
   for (Segment& sg : segs) {
@@ -471,7 +468,8 @@ int batch_run(const FieldSpec* specs, int64_t nfields, int profile, hipStream_t 
     const FieldSpec& s = specs[k];
     const size_t n = static_cast<size_t>(s.ny * s.nx);
     off_cells[k] = bytes;
-    if (s.iterations > 0) bytes = align_up(bytes + sizeof(Cell<T>) * n);
+    if (s.iterations > 0)
+      bytes = align_up(bytes + sizeof(Cell<T>) * static_cast<size_t>(guarded_cells(s.ny, s.nx)));
     off_tmp[k] = bytes;
     if (s.iterations > 1) bytes = align_up(bytes + sizeof(T) * n);
     off_codes[k] = bytes;

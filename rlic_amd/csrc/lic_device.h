@@ -37,11 +37,11 @@ struct __align__(32) Cell<double> {
 template <typename T>
 struct CellVal {
   T u, v, x;
-  T pad;  // a guarded array's jump bits (see pad_bits)
+  T pad;  // a guarded array's jump (see pad_bits)
 };
 
-// The byte jump a guarded cell array keeps in a cell's pad slot (low 32
-// bits of its bit pattern; 0 in image cells).
+// The jump in cells that a guarded cell array keeps in a cell's pad slot
+// (low 32 bits of its bit pattern; 0 in image cells).
 __device__ __forceinline__ uint32_t pad_bits(float pad) { return __builtin_bit_cast(uint32_t, pad); }
 __device__ __forceinline__ uint32_t pad_bits(double pad) {
   return static_cast<uint32_t>(__builtin_bit_cast(uint64_t, pad));
@@ -230,17 +230,60 @@ struct FieldGeom {
   int32_t x_left, x_right, y_left, y_right;  // 1 = periodic, 0 = closed
 };
 
-// Per-field constants of the incremental step rule.  A position is kept as
-// (row_off, j) with row_off = i * nx, so the cell index is row_off + j and a
-// step changes one of the two by +-1 / +-nx, or, stepping off an edge, the
-// boundary rule applies (src/lib.rs:83-95): a closed edge keeps the edge
-// cell, a periodic edge wraps to the opposite edge cell.  At an edge the
-// coordinate is 0 or its maximum m, so the wrapped value is m - c.
+// Per-field constant of the incremental step rule on a guarded cell array
+// (positions in `unit`s): the row step.
 struct StepGeom {
-  uint32_t nx, nxm1, last_row;  // row step, last column, last row offset
-  bool x_lo_per, x_hi_per, y_lo_per, y_hi_per;  // periodic sides
-  uint32_t grow;  // row step of a guarded cell array ((ny + 2) x (nx + 2))
+  uint32_t grow;
 };
+
+// Cells of the guarded array of an ny x nx field: (ny + 2) x (nx + 2), the
+// image's cell (i, j) at (i + 1, j + 1), a ring of guard cells around it.
+__host__ __device__ inline int64_t guarded_cells(int64_t ny, int64_t nx) {
+  return (ny + 2) * (nx + 2);
+}
+
+// Image cell (ti, tj) whose values guard cell (gi, gj) of the guarded array
+// holds: itself for an image cell, else the cell the boundary rule sends a
+// step onto the guard cell to (src/lib.rs:83-95: the edge cell on a closed
+// side, the opposite edge cell on a periodic one).  False for the four
+// corners, which no step reaches (a step changes one coordinate).
+__host__ __device__ inline bool guard_source(int32_t gi, int32_t gj, int32_t ny, int32_t nx,
+                                             int32_t x_left, int32_t x_right, int32_t y_left,
+                                             int32_t y_right, int32_t& ti, int32_t& tj) {
+  ti = gi - 1;
+  tj = gj - 1;
+  const bool out_y = ti < 0 || ti >= ny, out_x = tj < 0 || tj >= nx;
+  if (out_y && out_x) return false;
+  if (tj < 0) tj = x_left ? nx - 1 : 0;
+  if (tj >= nx) tj = x_right ? 0 : nx - 1;
+  if (ti < 0) ti = y_left ? ny - 1 : 0;
+  if (ti >= ny) ti = y_right ? 0 : ny - 1;
+  return true;
+}
+
+// Cell (gi, gj) of a field's guarded array from its strided texture, u, v.
+template <typename T>
+__device__ __forceinline__ Cell<T> guarded_cell(const T* __restrict__ tex, int64_t tsy,
+                                                int64_t tsx, const T* __restrict__ u,
+                                                int64_t usy, int64_t usx,
+                                                const T* __restrict__ v, int64_t vsy,
+                                                int64_t vsx, const FieldGeom& g, int32_t gi,
+                                                int32_t gj) {
+  Cell<T> c;
+  int32_t i, j;
+  if (guard_source(gi, gj, g.ny, g.nx, g.x_left, g.x_right, g.y_left, g.y_right, i, j)) {
+    c.u = u[i * usy + j * usx];
+    c.v = v[i * vsy + j * vsx];
+    c.x = tex[i * tsy + j * tsx];
+    c.pad = jump_pad((i + 1 - gi) * (g.nx + 2) + (j + 1 - gj), static_cast<T*>(nullptr));
+  } else {
+    c.u = T(0);
+    c.v = T(0);
+    c.x = T(0);
+    c.pad = T(0);
+  }
+  return c;
+}
 
 // Move codes recorded by the tracing pass (4 bits per step, 8 per word), a
 // bit field: kMoveMoved for a step to a neighbouring cell, plus kMoveAxisY
@@ -274,17 +317,10 @@ __host__ __device__ inline bool group_has_stop(uint32_t word) {
   return (word >> 28) == kMoveStop;
 }
 
-// Positions are in `unit`s: 1 for element indices, the element size for
-// byte offsets.
+// Positions are in `unit`s: 1 for element indices, the cell size for byte
+// offsets.
 __host__ __device__ inline StepGeom make_step_geom(const FieldGeom& g, uint32_t unit) {
   StepGeom s;
-  s.nx = static_cast<uint32_t>(g.nx) * unit;
-  s.nxm1 = static_cast<uint32_t>(g.nx - 1) * unit;
-  s.last_row = static_cast<uint32_t>(g.ny - 1) * static_cast<uint32_t>(g.nx) * unit;
-  s.x_lo_per = g.x_left != 0;
-  s.x_hi_per = g.x_right != 0;
-  s.y_lo_per = g.y_left != 0;
-  s.y_hi_per = g.y_right != 0;
   s.grow = static_cast<uint32_t>(g.nx + 2) * unit;
   return s;
 }

@@ -1,6 +1,6 @@
 // HIP kernels of the line integral convolution hot path (gfx950).
 //
-//   lic_pack    strided (texture, u, v) -> packed Cell array (one pass, HBM-bound)
+//   lic_pack    strided (texture, u, v) -> guarded Cell array (one pass, HBM-bound)
 //   lic_trace   one lane per output pixel: centre tap, forward streamline,
 //               backward streamline (src/lib.rs:364-406, 304-362); optionally
 //               records the cell-to-cell move of every step
@@ -18,94 +18,36 @@
 
 namespace rlic {
 
+// The guarded cell array of a field (guarded_cells): the image's cells at
+// (i + 1, j + 1) and a ring of guard cells around them.  A step off the
+// image lands on a guard cell, which holds the values of the cell the
+// boundary rule sends that step to (guard_source) and, in its pad slot, the
+// jump in cells from the guard cell to that cell (0 in image cells).
+// Corner cells are never reached and stay zero.
 template <typename T>
 __global__ __launch_bounds__(256) void lic_pack(const T* __restrict__ tex, int64_t tsy,
                                                 int64_t tsx, const T* __restrict__ u,
                                                 int64_t usy, int64_t usx,
                                                 const T* __restrict__ v, int64_t vsy,
-                                                int64_t vsx, int32_t ny, int32_t nx,
+                                                int64_t vsx, FieldGeom g,
                                                 Cell<T>* __restrict__ cells) {
-  const int32_t j = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int32_t i = blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (i >= ny || j >= nx) return;
-  Cell<T> c;
-  c.u = u[i * usy + j * usx];
-  c.v = v[i * vsy + j * vsx];
-  c.x = tex[i * tsy + j * tsx];
-  c.pad = T(0);
-  cells[static_cast<int64_t>(i) * nx + j] = c;
-}
-
-// The guarded cell array of the single-field trace: (ny + 2) x (nx + 2)
-// cells, the image's at (i + 1, j + 1) and a ring of guard cells around
-// them.  A step off the image lands on a guard cell; the guard cell holds
-// the values of the cell the boundary rule sends that step to
-// (src/lib.rs:83-95: the edge cell itself on a closed side, the opposite
-// edge cell on a periodic one) and, in its pad slot, the byte jump from
-// the guard cell to that cell (0 in image cells).  Corner cells are never
-// reached (a step changes one coordinate) and stay zero.
-//
-// Image cell (ti, tj) of guarded cell (gi, gj); false for the corners.
-__device__ __forceinline__ bool guard_source(int32_t gi, int32_t gj, int32_t ny, int32_t nx,
-                                             const FieldGeom& g, int32_t& ti, int32_t& tj) {
-  ti = gi - 1;
-  tj = gj - 1;
-  const bool out_y = ti < 0 || ti >= ny, out_x = tj < 0 || tj >= nx;
-  if (out_y && out_x) return false;
-  if (tj < 0) tj = g.x_left ? nx - 1 : 0;
-  if (tj >= nx) tj = g.x_right ? 0 : nx - 1;
-  if (ti < 0) ti = g.y_left ? ny - 1 : 0;
-  if (ti >= ny) ti = g.y_right ? 0 : ny - 1;
-  return true;
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void lic_pack_guarded(const T* __restrict__ tex, int64_t tsy,
-                                                        int64_t tsx, const T* __restrict__ u,
-                                                        int64_t usy, int64_t usx,
-                                                        const T* __restrict__ v, int64_t vsy,
-                                                        int64_t vsx, FieldGeom g,
-                                                        Cell<T>* __restrict__ cells) {
-  const int32_t ny = g.ny, nx = g.nx;
   const int32_t gj = blockIdx.x * 64 + (threadIdx.x & 63);
   const int32_t gi = blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (gi >= ny + 2 || gj >= nx + 2) return;
-  Cell<T> c;
-  int32_t i, j;
-  if (guard_source(gi, gj, ny, nx, g, i, j)) {
-    c.u = u[i * usy + j * usx];
-    c.v = v[i * vsy + j * vsx];
-    c.x = tex[i * tsy + j * tsx];
-    const int32_t jump = ((i + 1 - gi) * (nx + 2) + (j + 1 - gj)) * static_cast<int32_t>(sizeof(Cell<T>));
-    c.pad = jump_pad(jump, static_cast<T*>(nullptr));
-  } else {
-    c.u = T(0);
-    c.v = T(0);
-    c.x = T(0);
-    c.pad = T(0);
-  }
-  cells[static_cast<int64_t>(gi) * (nx + 2) + gj] = c;
+  if (gi >= g.ny + 2 || gj >= g.nx + 2) return;
+  cells[static_cast<int64_t>(gi) * (g.nx + 2) + gj] =
+      guarded_cell(tex, tsy, tsx, u, usy, usx, v, vsy, vsx, g, gi, gj);
 }
 
-// The texture slot of every cell from the C-order image x (guarded: every
-// cell of the guarded array, guard cells from their boundary-rule source).
+// The texture slot of every cell of the guarded array from the C-order
+// image x (guard cells from their boundary-rule source).
 template <typename T>
 __global__ __launch_bounds__(256) void lic_repack_texture(const T* __restrict__ x, FieldGeom g,
                                                           Cell<T>* __restrict__ cells) {
-  const int64_t p = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-  if (p >= static_cast<int64_t>(g.ny) * g.nx) return;
-  cells[p].x = x[p];
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void lic_repack_texture_guarded(const T* __restrict__ x,
-                                                                  FieldGeom g,
-                                                                  Cell<T>* __restrict__ cells) {
   const int32_t gj = blockIdx.x * 64 + (threadIdx.x & 63);
   const int32_t gi = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (gi >= g.ny + 2 || gj >= g.nx + 2) return;
   int32_t i, j;
-  if (guard_source(gi, gj, g.ny, g.nx, g, i, j))
+  if (guard_source(gi, gj, g.ny, g.nx, g.x_left, g.x_right, g.y_left, g.y_right, i, j))
     cells[static_cast<int64_t>(gi) * (g.nx + 2) + gj].x = x[static_cast<int64_t>(i) * g.nx + j];
 }
 
@@ -130,38 +72,33 @@ __global__ __launch_bounds__(256) void lic_copy2d(const T* __restrict__ src, int
   dst[static_cast<int64_t>(i) * nx + j] = src[i * sy + j * sx];
 }
 
-// With BUF (buffer loads, arrays below 4 GiB) the cells are the guarded
-// array of lic_pack_guarded and a position is one byte offset into it;
-// otherwise the plain array of lic_pack, rows and columns kept apart.
+// The cells are the field's guarded array (lic_pack); a position is a
+// byte offset into it with BUF (buffer loads, arrays below 4 GiB), else a
+// cell index.
 template <typename T, bool FMA, bool BRANCHLESS, bool POL, bool REC, bool BUF>
 __global__ __launch_bounds__(256) void lic_trace(const Cell<T>* __restrict__ cells,
                                                  const T* __restrict__ kernel, int32_t klen,
                                                  FieldGeom g, TileMap m, T* __restrict__ out,
                                                  uint4* __restrict__ codes) {
-  constexpr bool GUARD = BUF;
   int32_t ty, tx, py, px;
   tile_of_block(blockIdx.x, m, ty, tx);
   pixel_of_thread(threadIdx.x, m, py, px);
   const int32_t i0 = ty * tile_h(m) + py, j0 = tx * tile_w(m) + px;
   if (i0 >= g.ny || j0 >= g.nx) return;
-  const int64_t ncells = GUARD ? static_cast<int64_t>(g.ny + 2) * (g.nx + 2)
-                               : static_cast<int64_t>(g.ny) * g.nx;
-  const CellGather<T, BUF> cell(cells, ncells);
+  const CellGather<T, BUF> cell(cells, guarded_cells(g.ny, g.nx));
   const StepGeom sg = make_step_geom(g, cell.unit);
   const int32_t kmid = klen / 2;
   const int32_t n_fwd = klen - 1 - kmid, n_bwd = kmid;
   const int64_t np = static_cast<int64_t>(g.ny) * g.nx;  // code slot = pixel index
   const int64_t p = static_cast<int64_t>(i0) * g.nx + j0;
-  const uint32_t row0 = GUARD ? static_cast<uint32_t>(i0 + 1) * sg.grow +
-                                    static_cast<uint32_t>(j0 + 1) * cell.unit
-                              : static_cast<uint32_t>(i0) * sg.nx;
-  const uint32_t jj0 = GUARD ? 0u : static_cast<uint32_t>(j0) * cell.unit;
-  const CellVal<T> c0 = cell(row0 + jj0);
+  const uint32_t row0 =
+      static_cast<uint32_t>(i0 + 1) * sg.grow + static_cast<uint32_t>(j0 + 1) * cell.unit;
+  const CellVal<T> c0 = cell(row0);
   T acc = macc<FMA>(kernel[kmid], c0.x, T(0));  // src/lib.rs:375-383
-  acc = trace_direction<T, FMA, BRANCHLESS, POL, true, REC, BUF, GUARD>(
-      acc, c0.u, c0.v, row0, jj0, cell, kernel, kmid, n_fwd, sg, codes, np, p, 0);
-  acc = trace_direction<T, FMA, BRANCHLESS, POL, false, REC, BUF, GUARD>(
-      acc, c0.u, c0.v, row0, jj0, cell, kernel, kmid, n_bwd, sg, codes, np, p,
+  acc = trace_direction<T, FMA, BRANCHLESS, POL, true, REC, BUF>(
+      acc, c0.u, c0.v, row0, cell, kernel, kmid, n_fwd, sg, codes, np, p, 0);
+  acc = trace_direction<T, FMA, BRANCHLESS, POL, false, REC, BUF>(
+      acc, c0.u, c0.v, row0, cell, kernel, kmid, n_bwd, sg, codes, np, p,
       chunks_of(n_fwd));
   out[static_cast<int64_t>(i0) * g.nx + j0] = acc;
 }
@@ -680,17 +617,11 @@ int32_t code_chunks(int32_t klen) {
 
 template <typename T>
 hipError_t launch_pack(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t usy,
-                       int64_t usx, const T* v, int64_t vsy, int64_t vsx, int32_t ny,
-                       int32_t nx, Cell<T>* cells, hipStream_t stream, const FieldGeom* guarded) {
-  if (guarded) {
-    dim3 grid((nx + 2 + 63) / 64, (ny + 2 + 3) / 4);
-    hipLaunchKernelGGL(lic_pack_guarded<T>, grid, dim3(256), 0, stream, tex, tsy, tsx, u, usy,
-                       usx, v, vsy, vsx, *guarded, cells);
-  } else {
-    dim3 grid((nx + 63) / 64, (ny + 3) / 4);
-    hipLaunchKernelGGL(lic_pack<T>, grid, dim3(256), 0, stream, tex, tsy, tsx, u, usy, usx, v,
-                       vsy, vsx, ny, nx, cells);
-  }
+                       int64_t usx, const T* v, int64_t vsy, int64_t vsx, const FieldGeom& g,
+                       Cell<T>* cells, hipStream_t stream) {
+  dim3 grid((g.nx + 2 + 63) / 64, (g.ny + 2 + 3) / 4);
+  hipLaunchKernelGGL(lic_pack<T>, grid, dim3(256), 0, stream, tex, tsy, tsx, u, usy, usx, v, vsy,
+                     vsx, g, cells);
   return hipGetLastError();
 }
 
@@ -707,17 +638,11 @@ hipError_t launch_pad_kernel(const T* kernel, int32_t klen, T* dst, hipStream_t 
 }
 
 template <typename T>
-hipError_t launch_repack_texture(const T* x, const FieldGeom& g, bool guarded, Cell<T>* cells,
+hipError_t launch_repack_texture(const T* x, const FieldGeom& g, Cell<T>* cells,
                                  hipStream_t stream) {
-  const int64_t n = static_cast<int64_t>(g.ny) * g.nx;
-  if (n == 0) return hipSuccess;
-  if (guarded) {
-    dim3 grid((g.nx + 2 + 63) / 64, (g.ny + 2 + 3) / 4);
-    hipLaunchKernelGGL(lic_repack_texture_guarded<T>, grid, dim3(256), 0, stream, x, g, cells);
-  } else {
-    hipLaunchKernelGGL(lic_repack_texture<T>, dim3(static_cast<unsigned>((n + 255) / 256)),
-                       dim3(256), 0, stream, x, g, cells);
-  }
+  if (static_cast<int64_t>(g.ny) * g.nx == 0) return hipSuccess;
+  dim3 grid((g.nx + 2 + 63) / 64, (g.ny + 2 + 3) / 4);
+  hipLaunchKernelGGL(lic_repack_texture<T>, grid, dim3(256), 0, stream, x, g, cells);
   return hipGetLastError();
 }
 
@@ -732,17 +657,12 @@ hipError_t launch_copy2d(const T* src, int64_t sy, int64_t sx, int32_t ny, int32
 // Buffer addressing covers arrays of fewer than 2^32 bytes.
 inline bool fits_buffer(int64_t bytes) { return bytes > 0 && bytes <= 0xFFFFFFFFll; }
 
-template <typename T>
-bool trace_guarded(int32_t ny, int32_t nx) {
-  return fits_buffer(static_cast<int64_t>(ny + 2) * (nx + 2) *
-                     static_cast<int64_t>(sizeof(Cell<T>)));
-}
 
 template <typename T, bool FMA, bool BL, bool POL, bool REC>
 static hipError_t trace_one(const Cell<T>* cells, const T* kernel, int32_t klen,
                             const FieldGeom& g, const TileMap& m, T* out, uint4* codes,
                             hipStream_t stream) {
-  if (trace_guarded<T>(g.ny, g.nx)) {  // buffer loads on the guarded array
+  if (fits_buffer(guarded_cells(g.ny, g.nx) * static_cast<int64_t>(sizeof(Cell<T>)))) {
     hipLaunchKernelGGL((lic_trace<T, FMA, BL, POL, REC, true>), dim3(m.n_tiles), dim3(kBlock),
                        0, stream, cells, kernel, klen, g, m, out, codes);
   } else {
@@ -950,11 +870,10 @@ hipError_t launch_replay_lds_hybrid(const T* x, const T* kernel, int32_t klen, c
 
 #define RLIC_INSTANTIATE(T)                                                                     \
   template hipError_t launch_pack<T>(const T*, int64_t, int64_t, const T*, int64_t, int64_t,   \
-                                     const T*, int64_t, int64_t, int32_t, int32_t, Cell<T>*,   \
-                                     hipStream_t, const FieldGeom*);                           \
-  template hipError_t launch_repack_texture<T>(const T*, const FieldGeom&, bool, Cell<T>*,     \
+                                     const T*, int64_t, int64_t, const FieldGeom&, Cell<T>*,   \
+                                     hipStream_t);                                             \
+  template hipError_t launch_repack_texture<T>(const T*, const FieldGeom&, Cell<T>*,           \
                                                hipStream_t);                                    \
-  template bool trace_guarded<T>(int32_t, int32_t);                                             \
   template hipError_t launch_pad_kernel<T>(const T*, int32_t, T*, hipStream_t);                \
   template hipError_t launch_copy2d<T>(const T*, int64_t, int64_t, int32_t, int32_t, T*,       \
                                        hipStream_t);                                           \

@@ -15,18 +15,12 @@ TileMap make_tile_map(int32_t ny, int32_t nx, int32_t band, int32_t wave_w_log2)
 // slot for a kernel of length klen (forward chunks, then backward chunks).
 int32_t code_chunks(int32_t klen);
 
+// The field's guarded cell array (guarded_cells(ny, nx) cells) from its
+// strided texture, u and v.
 template <typename T>
 hipError_t launch_pack(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t usy,
-                       int64_t usx, const T* v, int64_t vsy, int64_t vsx, int32_t ny,
-                       int32_t nx, Cell<T>* cells, hipStream_t stream,
-                       const FieldGeom* guarded = nullptr);
-
-// The single-field trace's cell layout: guarded ((ny + 2) x (nx + 2) cells,
-// a ring of guard cells around the image, launch_pack given the field's
-// geometry)
-// when that array is buffer-addressable (< 4 GiB), else plain (ny x nx).
-template <typename T>
-bool trace_guarded(int32_t ny, int32_t nx);
+                       int64_t usx, const T* v, int64_t vsy, int64_t vsx, const FieldGeom& g,
+                       Cell<T>* cells, hipStream_t stream);
 
 // Zeros on each side of the padded kernel-weight copy the streamline
 // kernels read (they load taps in groups of 8 without bounds checks).
@@ -37,7 +31,7 @@ template <typename T>
 hipError_t launch_pad_kernel(const T* kernel, int32_t klen, T* dst, hipStream_t stream);
 
 template <typename T>
-hipError_t launch_repack_texture(const T* x, const FieldGeom& g, bool guarded, Cell<T>* cells,
+hipError_t launch_repack_texture(const T* x, const FieldGeom& g, Cell<T>* cells,
                                  hipStream_t stream);
 
 template <typename T>

@@ -45,7 +45,7 @@ template <typename T>
 size_t Server::field_bytes(int64_t ny, int64_t nx, int64_t klen, bool record) {
   const size_t n = static_cast<size_t>(ny * nx);
   size_t b = align256(sizeof(T) * static_cast<size_t>(std::max<int64_t>(klen, 0) + 2 * kernel_pad()));
-  b += align256(sizeof(Cell<T>) * n);
+  b += align256(sizeof(Cell<T>) * static_cast<size_t>(guarded_cells(ny, nx)));
   b += 2 * align256(sizeof(T) * n);
   if (record) b += align256(sizeof(uint4) * static_cast<size_t>(code_chunks(static_cast<int32_t>(klen))) * n);
   return b;
@@ -80,7 +80,7 @@ int Server::admit(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
   r.weights = padded + pad;
   p += align256(sizeof(T) * static_cast<size_t>(std::max<int64_t>(klen, 0) + 2 * pad));
   r.cells = reinterpret_cast<Cell<T>*>(p);
-  p += align256(sizeof(Cell<T>) * n);
+  p += align256(sizeof(Cell<T>) * static_cast<size_t>(guarded_cells(ny, nx)));
   r.buf[0] = reinterpret_cast<T*>(p);
   p += align256(sizeof(T) * n);
   r.buf[1] = reinterpret_cast<T*>(p);
@@ -95,8 +95,7 @@ int Server::admit(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
     e = hipMemcpyAsync(r.weights, kernel, sizeof(T) * static_cast<size_t>(klen),
                        hipMemcpyDefault, stream_);
   if (e == hipSuccess && n > 0)
-    e = launch_pack<T>(tex, tsy, tsx, u, usy, usx, v, vsy, vsx, static_cast<int32_t>(ny),
-                       static_cast<int32_t>(nx), r.cells, stream_);
+    e = launch_pack<T>(tex, tsy, tsx, u, usy, usx, v, vsy, vsx, r.g, r.cells, stream_);
   if (e != hipSuccess) {
     (void)hipFreeAsync(r.block, stream_);
     err = std::string("admitting field: ") + hipGetErrorString(e);

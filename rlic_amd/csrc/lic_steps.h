@@ -33,18 +33,17 @@ __host__ __device__ inline int32_t chunks_of(int32_t nsteps) {
 // POL and REC: 0 = off, 1 = on, kRuntime = decided by the wave-uniform
 // arguments pol_on / rec_on (one kernel body for fields of mixed modes).
 //
-// GUARD: the cells form a guarded array, (ny + 2) x (nx + 2) with a ring of
-// guard cells around the image, and `row` carries the whole position (`j`
-// is unused).  A guard cell holds the values of the cell that a step onto
-// it ends on by the boundary rule (src/lib.rs:83-95: the edge cell itself
-// for a closed side, the opposite edge cell for a periodic one) and, in its
-// pad slot, the byte jump to that cell (0 in image cells): a step is one
-// add of +-1 cell or +-1 row, the gather, and one add of the jump.
+// The cells form a guarded array (make_guarded_cells: (ny + 2) x (nx + 2)
+// with a ring of guard cells around the image) and `row` is the position.
+// A guard cell holds the values of the cell that a step onto it ends on by
+// the boundary rule (src/lib.rs:83-95: the edge cell itself for a closed
+// side, the opposite edge cell for a periodic one) and, in its pad slot,
+// the jump in cells to that cell (0 in image cells): a step is one add of
+// +-1 cell or +-1 row, the gather, and one shifted add of the jump.
 constexpr int kRuntime = 2;
 
-template <typename T, bool FMA, bool BRANCHLESS, int POL, bool FORWARD, int REC, bool BUF,
-          bool GUARD = false>
-__device__ __forceinline__ T trace_direction(T acc, T pu, T pv, uint32_t row, uint32_t j,
+template <typename T, bool FMA, bool BRANCHLESS, int POL, bool FORWARD, int REC, bool BUF>
+__device__ __forceinline__ T trace_direction(T acc, T pu, T pv, uint32_t row,
                                              const CellGather<T, BUF>& cell,
                                              const T* __restrict__ kernel, int32_t kmid,
                                              int32_t nsteps, const StepGeom& sg,
@@ -57,16 +56,8 @@ __device__ __forceinline__ T trace_direction(T acc, T pu, T pv, uint32_t row, ui
   T last_u = T(0), last_v = T(0);  // src/lib.rs:320-323
   bool alive = true;
   const uint32_t unit = cell.unit;
-  // select operands: edge positions, edge targets (src/lib.rs:83-95: a
-  // closed side keeps the edge cell, a periodic side wraps to the opposite
-  // edge cell) and row steps, held in VGPRs across the loop
-  const uint32_t nxm1 = lane_copy(sg.nxm1), last_row = lane_copy(sg.last_row);
-  const uint32_t x_hi = lane_copy(sg.x_hi_per ? 0u : sg.nxm1);
-  const uint32_t x_lo = lane_copy(sg.x_lo_per ? sg.nxm1 : 0u);
-  const uint32_t y_hi = lane_copy(sg.y_hi_per ? 0u : sg.last_row);
-  const uint32_t y_lo = lane_copy(sg.y_lo_per ? sg.last_row : 0u);
-  const uint32_t row_up = lane_copy(GUARD ? sg.grow : sg.nx);
-  const uint32_t row_down = lane_copy(0u - (GUARD ? sg.grow : sg.nx));
+  const uint32_t row_up = lane_copy(sg.grow);  // row step, held in VGPRs across the loop
+  const uint32_t row_down = lane_copy(0u - sg.grow);
   const int32_t nchunks = chunks_of(nsteps);
   for (int32_t ch = 0; ch < nchunks; ++ch) {
     uint32_t words[4] = {0u, 0u, 0u, 0u};
@@ -118,29 +109,12 @@ __device__ __forceinline__ T trace_direction(T acc, T pu, T pv, uint32_t row, ui
         const T fy_new = xs ? fy_after_x : (pos_y ? T(0) : T(1));
         fx = moves ? fx_new : fx;
         fy = moves ? fy_new : fy;
-        bool edge_x = false, edge_y = false, edge = false;
-        CellVal<T> cv;
-        if constexpr (GUARD) {
-          const uint32_t step = xs ? (pos_x ? unit : 0u - unit) : (pos_y ? row_up : row_down);
-          const uint32_t at = moves ? row + step : row;
-          cv = cell(at);
-          const uint32_t jump = pad_bits(cv.pad);
-          edge = jump != 0u;  // the step left the image
-          row = at + jump;
-        } else {
-          // the crossed coordinate with the boundary rule (src/lib.rs:83-95):
-          // a step off an edge leaves [0, last] (unsigned, so stepping below 0
-          // wraps to a huge value), and the side's rule picks the target
-          const uint32_t j_step = j + (pos_x ? unit : 0u - unit);
-          edge_x = j_step > nxm1;
-          const uint32_t j_to = edge_x ? (pos_x ? x_hi : x_lo) : j_step;
-          const uint32_t row_step = row + (pos_y ? row_up : row_down);
-          edge_y = row_step > last_row;
-          const uint32_t row_to = edge_y ? (pos_y ? y_hi : y_lo) : row_step;
-          j = (moves && xs) ? j_to : j;
-          row = (moves && !xs) ? row_to : row;
-          cv = cell(row + j);
-        }
+        const uint32_t step = xs ? (pos_x ? unit : 0u - unit) : (pos_y ? row_up : row_down);
+        const uint32_t at = moves ? row + step : row;
+        const CellVal<T> cv = cell(at);
+        const uint32_t jump = pad_bits(cv.pad);
+        const bool edge = jump != 0u;  // the step left the image
+        row = at + jump * unit;
         const T next_acc = macc<FMA>(kw[q], cv.x, acc);  // src/lib.rs:353-360
         acc = live ? next_acc : acc;
         pu = cv.u;
@@ -157,10 +131,7 @@ __device__ __forceinline__ T trace_direction(T acc, T pu, T pv, uint32_t row, ui
           const uint64_t m_nz = __builtin_amdgcn_ballot_w64(mu != T(0)) |
                                 __builtin_amdgcn_ballot_w64(mv != T(0));
           const uint64_t m_x = __builtin_amdgcn_ballot_w64(xs);
-          const uint64_t m_edge =
-              GUARD ? __builtin_amdgcn_ballot_w64(edge)
-                    : (m_x & __builtin_amdgcn_ballot_w64(edge_x)) |
-                          (~m_x & __builtin_amdgcn_ballot_w64(edge_y));
+          const uint64_t m_edge = __builtin_amdgcn_ballot_w64(edge);
           const uint64_t m_neg = ~((m_x & __builtin_amdgcn_ballot_w64(pos_x)) |
                                    (~m_x & __builtin_amdgcn_ballot_w64(pos_y)));
           uint32_t code = lane_bit(m_nz & ~m_stop);

@@ -47,12 +47,13 @@ def _submit(server, r, **kw):
 
 def test_field_bytes_arithmetic():
     lib = _native.load()
-    # 64 x 64 f32, L = 33: weights 256 + cells 64 KiB + two buffers 16 KiB each
-    assert lib.rlic_server_field_bytes_f32(64, 64, 33, 0, 1) == 256 + 65536 + 2 * 16384
+    # 64 x 64 f32, L = 33: weights 256 + guarded cells (66 x 66 x 16 B, 256-aligned)
+    # + two buffers 16 KiB each
+    assert lib.rlic_server_field_bytes_f32(64, 64, 33, 0, 1) == 256 + 69888 + 2 * 16384
     # keeping the move record adds 2 chunks (16 + 16 steps) of 16 B per pixel
-    assert lib.rlic_server_field_bytes_f32(64, 64, 33, 1, 1) == 256 + 65536 + 2 * 16384 + 131072
-    assert lib.rlic_server_field_bytes_f32(64, 64, 33, 0, 2) == 256 + 65536 + 2 * 16384 + 131072
-    assert lib.rlic_server_field_bytes_f64(10, 10, 1, 0, 1) == 256 + 3328 + 2 * 1024
+    assert lib.rlic_server_field_bytes_f32(64, 64, 33, 1, 1) == 256 + 69888 + 2 * 16384 + 131072
+    assert lib.rlic_server_field_bytes_f32(64, 64, 33, 0, 2) == 256 + 69888 + 2 * 16384 + 131072
+    assert lib.rlic_server_field_bytes_f64(10, 10, 1, 0, 1) == 256 + 4608 + 2 * 1024
     assert lib.rlic_server_field_bytes_f32(-1, 4, 3, 0, 1) == -1
 
 
