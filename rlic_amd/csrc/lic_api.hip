@@ -3,8 +3,9 @@
 // Iteration driver (src/lib.rs:408-443, convolve_iteratively): the
 // reference copies the texture into a C-order buffer, runs `iterations`
 // passes and swaps input/output between passes.  Here:
-//   1. lic_pack builds the packed (u, v, texture) cell array from the
-//      caller's strided arrays (any strides, including transposed views);
+//   1. lic_pack builds the guarded (u, v, texture) cell array, a ring of
+//      boundary-rule guard cells around the image, from the caller's
+//      strided arrays (any strides, including transposed views);
 //   2. lic_trace runs pass 1 and, when more passes follow, records the move
 //      of every streamline step;
 //   3. lic_replay runs passes 2..n from the recorded moves, ping-ponging

@@ -33,7 +33,7 @@ __host__ __device__ inline int32_t chunks_of(int32_t nsteps) {
 // POL and REC: 0 = off, 1 = on, kRuntime = decided by the wave-uniform
 // arguments pol_on / rec_on (one kernel body for fields of mixed modes).
 //
-// The cells form a guarded array (make_guarded_cells: (ny + 2) x (nx + 2)
+// The cells form a guarded array (guarded_cells: (ny + 2) x (nx + 2)
 // with a ring of guard cells around the image) and `row` is the position.
 // A guard cell holds the values of the cell that a step onto it ends on by
 // the boundary rule (src/lib.rs:83-95: the edge cell itself for a closed
