@@ -170,17 +170,24 @@ def test_strided_inputs():
     _assert_same(got, exp, "broadcast")
 
 
-def test_retrace_path_matches_replay():
-    """Iterations re-traced every pass (no move record) give the same bits."""
+@pytest.mark.parametrize("boundaries", [
+    "closed",
+    "periodic",
+    {"x": "periodic", "y": "closed"},
+])
+def test_retrace_path_matches_replay(boundaries):
+    """Iterations re-traced every pass (no move record) give the same bits;
+    between passes the texture of every cell is refreshed, including the
+    guard cells that stand for the boundary-rule targets."""
     rng = np.random.default_rng(11)
     tex, u, v = _fields(rng, (50, 61), np.float32, "holes")
     k = rng.random(21).astype(np.float32)
-    exp = cpu.convolve(tex, u, v, kernel=k, uv_mode="polarization", iterations=4)
-    got_replay = rlic_amd.convolve(tex, u, v, kernel=k, uv_mode="polarization", iterations=4)
+    kw = dict(kernel=k, uv_mode="polarization", iterations=4, boundaries=boundaries)
+    exp = cpu.convolve(tex, u, v, **kw)
+    got_replay = rlic_amd.convolve(tex, u, v, **kw)
     os.environ["RLIC_REPLAY"] = "0"
     try:
-        got_retrace = rlic_amd.convolve(tex, u, v, kernel=k, uv_mode="polarization",
-                                        iterations=4)
+        got_retrace = rlic_amd.convolve(tex, u, v, **kw)
     finally:
         del os.environ["RLIC_REPLAY"]
     _assert_same(got_replay, exp, "replay")
