@@ -510,8 +510,11 @@ int run_on_device(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
         rlic::launch_pack<T>(tex, tsy, tsx, u, usy, usx, v, vsy, vsx, g, c, stream),
         "pack");
   }
-  const rlic::TileMap m = rlic::make_tile_map(ny, nx, env_int("RLIC_TILE_BAND", 16),
-                                              env_int("RLIC_WAVE_W_LOG2", 3));
+  // trace tiles of 32 x 8 pixels (16 x 4 per wave) in XCD bands of 64
+  // tiles: 2.57 vs 2.65 ms per trace launch at 4096^2, L=129 (sweep of
+  // bands 4..64, wave widths 8..32 pixels)
+  const rlic::TileMap m = rlic::make_tile_map(ny, nx, env_int("RLIC_TILE_BAND", 64),
+                                              env_int("RLIC_WAVE_W_LOG2", 4));
   if (iterations == 1 && !stats) {
     LaunchTimer timer("lic_trace", stream);
     RLIC_HIP_TRY(rlic::launch_trace<T>(c, kernel, klen, g, m, uv_mode, profile, out, nullptr,
