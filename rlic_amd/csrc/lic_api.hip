@@ -600,6 +600,28 @@ int run_on_device(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
   return RLIC_OK;
 }
 
+// Size limits of a non-empty image (ny, nx >= 1), checked before any
+// product is formed: each side and the pixel count below 2^31 (positions
+// and pixel indices are 32-bit), and the guarded cell array of
+// (ny + 2) x (nx + 2) cells addressable by a 32-bit cell index (thin images,
+// e.g. 1 x N with N > 1,431,655,763, pass the pixel limit but not this one).
+int check_size(int64_t ny, int64_t nx, int64_t klen) {
+  if (ny > INT32_MAX || nx > INT32_MAX || ny * nx > INT32_MAX)
+    return set_error(RLIC_ERR_INVALID_ARGUMENT,
+                     "image of %lld x %lld pixels exceeds the supported 2^31-1 pixels",
+                     static_cast<long long>(ny), static_cast<long long>(nx));
+  if (rlic::guarded_cells(ny, nx) > int64_t{UINT32_MAX})
+    return set_error(RLIC_ERR_INVALID_ARGUMENT,
+                     "image of %lld x %lld pixels exceeds the supported size: its guarded "
+                     "array of %lld x %lld cells exceeds 2^32-1 cells",
+                     static_cast<long long>(ny), static_cast<long long>(nx),
+                     static_cast<long long>(ny + 2), static_cast<long long>(nx + 2));
+  if (klen > INT32_MAX)
+    return set_error(RLIC_ERR_INVALID_ARGUMENT, "kernel of %lld weights exceeds 2^31-1",
+                     static_cast<long long>(klen));
+  return RLIC_OK;
+}
+
 // Argument checks shared by the single and batched entry points.
 template <typename T>
 int validate(const T* tex, const T* u, const T* v, int64_t ny, int64_t nx, const T* kernel,
@@ -616,12 +638,8 @@ int validate(const T* tex, const T* u, const T* v, int64_t ny, int64_t nx, const
     if (b != RLIC_BOUND_CLOSED && b != RLIC_BOUND_PERIODIC)
       return set_error(RLIC_ERR_INVALID_ARGUMENT, "unknown boundary %d", b);
   }
-  const int64_t n = ny * nx;
-  if (n == 0) return RLIC_OK;
-  if (ny > INT32_MAX || nx > INT32_MAX || n > INT32_MAX || klen > INT32_MAX)
-    return set_error(RLIC_ERR_INVALID_ARGUMENT,
-                     "image of %lld x %lld pixels exceeds the supported 2^31-1 pixels",
-                     static_cast<long long>(ny), static_cast<long long>(nx));
+  if (ny == 0 || nx == 0) return RLIC_OK;  // nothing to compute (the reference loops zero times)
+  if (const int rc = check_size(ny, nx, klen); rc != RLIC_OK) return rc;
   if (!tex || !u || !v || !out) return set_error(RLIC_ERR_INVALID_ARGUMENT, "null array pointer");
   if (iterations > 0 && klen == 0)
     return set_error(RLIC_ERR_EMPTY_KERNEL,
@@ -671,6 +689,9 @@ int batch_entry(const FieldSpec* fields, int64_t nfields, int profile, void* str
     LaunchTimer timer("lic_batch", stream);
     for (int64_t k = 0; k < nfields && rc == RLIC_OK; ++k) {
       const FieldSpec& f = fields[k];
+      // an empty image has an empty output: no work, and its other side may
+      // be any size (no per-field state is sized from it)
+      if (f.ny == 0 || f.nx == 0) continue;
       if (single_min <= 0 || f.ny * f.nx < single_min) {
         rest.push_back(f);
         continue;
@@ -723,12 +744,9 @@ int convolve_entry(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t u
     return set_error(RLIC_ERR_INVALID_ARGUMENT, "unknown memory location %d", location);
   if (profile < 0 || profile > 3)
     return set_error(RLIC_ERR_INVALID_ARGUMENT, "unknown arithmetic profile %d", profile);
+  if (ny == 0 || nx == 0) return RLIC_OK;  // nothing to compute (the reference loops zero times)
+  if (const int rc = check_size(ny, nx, klen); rc != RLIC_OK) return rc;
   const int64_t n = ny * nx;
-  if (n == 0) return RLIC_OK;  // nothing to compute (the reference loops zero times)
-  if (ny > INT32_MAX || nx > INT32_MAX || n > INT32_MAX || klen > INT32_MAX)
-    return set_error(RLIC_ERR_INVALID_ARGUMENT,
-                     "image of %lld x %lld pixels exceeds the supported 2^31-1 pixels",
-                     static_cast<long long>(ny), static_cast<long long>(nx));
   if (!tex || !u || !v || !out)
     return set_error(RLIC_ERR_INVALID_ARGUMENT, "null array pointer");
   if (iterations > 0 && klen == 0)
@@ -797,16 +815,22 @@ int server_admit(rlic::Server* server, const FieldSpec* f, int keep, int64_t* ha
     return set_error(RLIC_ERR_INVALID_ARGUMENT, "a served field needs at least one pass");
   std::string err;
   const bool record = keep != 0 || f->iterations > 1;
+  // an empty image is held as 0 x 0 with no weights (its output is empty
+  // whatever its sides and kernel)
+  const bool empty = f->ny == 0 || f->nx == 0;
   const int st = server->admit<T>(f->texture, f->tex_sy, f->tex_sx, f->u, f->u_sy, f->u_sx,
-                                  f->v, f->v_sy, f->v_sx, f->ny, f->nx, f->kernel,
-                                  f->kernel_len, f->uv_mode, f->x_left, f->x_right, f->y_left,
-                                  f->y_right, f->iterations, record, handle, err);
+                                  f->v, f->v_sy, f->v_sx, empty ? 0 : f->ny, empty ? 0 : f->nx,
+                                  f->kernel, empty ? 0 : f->kernel_len, f->uv_mode, f->x_left,
+                                  f->x_right, f->y_left, f->y_right, f->iterations, record,
+                                  handle, err);
   return st == RLIC_OK ? st : set_error(st, "%s", err.c_str());
 }
 
 template <typename T>
 int64_t server_field_bytes(int64_t ny, int64_t nx, int64_t klen, int keep, int64_t iterations) {
   if (ny < 0 || nx < 0 || klen < 0) return -1;
+  if (ny == 0 || nx == 0) ny = nx = klen = 0;  // held as 0 x 0 (see server_admit)
+  else if (check_size(ny, nx, klen) != RLIC_OK) return -1;
   return static_cast<int64_t>(
       rlic::Server::field_bytes<T>(ny, nx, klen, keep != 0 || iterations > 1));
 }

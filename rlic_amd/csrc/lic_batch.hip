@@ -39,6 +39,9 @@ template <typename T>
 __global__ __launch_bounds__(256) void lic_pad_batch(const BatchField<T>* fields,
                                                      const T* const* kernels) {
   const BatchField<T>& f = fields[blockIdx.x];
+  // an empty image has no pixel to weight, and its kernel pointer was not
+  // checked (the host accepts any kernel argument of an empty field)
+  if (f.g.ny == 0 || f.g.nx == 0) return;
   const T* src = kernels[blockIdx.x];
   T* dst = const_cast<T*>(f.kernel) - kKernelPad;
   const int32_t total = f.klen + 2 * kKernelPad;
@@ -361,7 +364,10 @@ int launch_queue(std::vector<BatchField<T>>& fields, const std::vector<const T*>
     f.regions_x = (f.g.nx + region - 1) / region;
     f.regions = f.regions_x * ((f.g.ny + region - 1) / region);
     f.pack_block0 = pack_blocks;
-    if (pack) pack_blocks += static_cast<int64_t>((f.g.nx + 2 + 63) / 64) * ((f.g.ny + 2 + 3) / 4);
+    // an empty image has no cells to pack and nothing to copy: its arrays may
+    // be zero-size (null pointers), so it gets no pack blocks at all
+    if (pack && f.g.ny > 0 && f.g.nx > 0)
+      pack_blocks += static_cast<int64_t>((f.g.nx + 2 + 63) / 64) * ((f.g.ny + 2 + 3) / 4);
     f.kernel_off = pool_len + kKernelPad;
     pool_len += f.klen + 2 * kKernelPad;
     if (guarded_cells(f.g.ny, f.g.nx) * static_cast<int64_t>(sizeof(Cell<T>)) > 0xFFFFFFFFll)

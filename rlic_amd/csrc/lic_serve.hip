@@ -91,7 +91,7 @@ int Server::admit(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
   hipError_t e = hipMemsetAsync(padded, 0, sizeof(T) * pad, stream_);
   if (e == hipSuccess)
     e = hipMemsetAsync(r.weights + klen, 0, sizeof(T) * pad, stream_);
-  if (e == hipSuccess && klen > 0)
+  if (e == hipSuccess && klen > 0 && n > 0)  // an empty field's kernel is never read
     e = hipMemcpyAsync(r.weights, kernel, sizeof(T) * static_cast<size_t>(klen),
                        hipMemcpyDefault, stream_);
   if (e == hipSuccess && n > 0)

@@ -9,6 +9,9 @@ import pytest
 ROOT = Path(__file__).resolve().parents[1]
 if str(ROOT) not in sys.path:
     sys.path.insert(0, str(ROOT))
+TESTS = Path(__file__).resolve().parent
+if str(TESTS) not in sys.path:  # shared helpers (oracle_check)
+    sys.path.insert(0, str(TESTS))
 
 # The library replays images below 2^18 pixels with L2 gathers by default;
 # the test fields are small, so the session lets the LDS-box and hybrid

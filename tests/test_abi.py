@@ -78,6 +78,51 @@ def test_empty_image_needs_no_device():
     assert _call(nx=0, kernel_len=0) == 0
 
 
+@pytest.mark.parametrize(
+    "ny, nx, message",
+    [
+        # more than 2^31-1 pixels
+        (1 << 16, 1 << 16, "exceeds the supported 2^31-1 pixels"),
+        # a side beyond 2^31-1: rejected before ny * nx is formed
+        (1 << 40, 4, "exceeds the supported 2^31-1 pixels"),
+        # ny * nx would wrap to 0 in 64-bit arithmetic: still rejected, not empty
+        (1 << 32, 1 << 32, "exceeds the supported 2^31-1 pixels"),
+        # thin images under the pixel limit whose (ny+2) x (nx+2) guarded
+        # cell array exceeds 32-bit cell indices
+        (1, 1_431_655_764, "guarded array of 3 x 1431655766 cells"),
+        (2_000_000_000, 1, "guarded array of 2000000002 x 3 cells"),
+    ],
+)
+def test_size_limits_are_status_codes(ny, nx, message):
+    # pointers are never dereferenced: the checks run before any HIP call
+    assert _call(ny=ny, nx=nx) == _native.RLIC_ERR_INVALID_ARGUMENT
+    assert message in _native.last_error()
+
+
+def test_largest_thin_image_passes_the_size_checks():
+    # 1 x 1,431,655,763 has 3 x 1,431,655,765 = 4,294,967,295 guarded cells:
+    # accepted by the size checks, so the call gets as far as its null
+    # pointer check (no GPU work is started)
+    assert _call(ny=1, nx=1_431_655_763, out=None) == _native.RLIC_ERR_INVALID_ARGUMENT
+    assert "null array pointer" in _native.last_error()
+
+
+def test_empty_image_with_a_huge_side_is_empty():
+    # the reference's zero-trip loop: an empty image returns at once, whatever
+    # its other side
+    assert _call(ny=0, nx=1 << 62) == 0
+    assert _call(ny=1 << 62, nx=0, kernel=None) == 0
+
+
+def test_server_field_bytes_rejects_oversized_images():
+    lib = _native.load()
+    assert lib.rlic_server_field_bytes_f32(1, 1_431_655_764, 3, 0, 1) == -1
+    assert lib.rlic_server_field_bytes_f32(1 << 40, 4, 3, 0, 1) == -1
+    # empty images are held as 0 x 0 with no weights
+    assert lib.rlic_server_field_bytes_f32(0, 1 << 40, 1 << 40, 1, 5) == \
+        lib.rlic_server_field_bytes_f32(0, 0, 0, 1, 5)
+
+
 def test_python_binding_argument_errors():
     a = np.ones((4, 4))
     k = np.ones(3)

@@ -9,6 +9,7 @@ import pytest
 
 import rlic_amd
 from oracle import cpu
+from oracle_check import assert_matches_oracle, oracle_convolve, same_bits
 from rlic_amd import _native
 from rlic_amd.batch import Field, convolve_batch
 
@@ -105,6 +106,79 @@ def test_batch_uneven_workload_sample():
                                    uv_mode=f["uv_mode"], boundaries=f["boundaries"],
                                    iterations=f["iterations"])
         assert torch.equal(out, single), f"field {k}"
+
+
+@pytest.mark.gpu
+def test_batch_with_empty_fields(gpu_device):
+    """Empty images ((0, N) and (N, 0)) at 0-2 iterations, as NumPy arrays and
+    as zero-size device tensors (null data pointers), mixed with normal
+    fields: empty outputs, the normal fields' bits unchanged (the reference
+    loops zero times over an empty image, src/lib.rs:373-374)."""
+    import torch
+
+    rng = np.random.default_rng(21)
+    normal = _mixed_fields(rng, 6)
+    empties = []
+    for k, shape in enumerate([(0, 5), (5, 0), (0, 0), (0, 37), (23, 0), (0, 1)]):
+        empties.append(Field(np.ones(shape), np.ones(shape), np.ones(shape), rng.random(7),
+                             uv_mode=("velocity", "polarization")[k % 2],
+                             boundaries=("closed", "periodic")[k % 3 == 0], iterations=k % 3))
+    fields = [f for pair in zip(normal, empties) for f in pair]
+    outs = convolve_batch(fields)
+    for k, (f, out) in enumerate(zip(fields, outs)):
+        assert out.shape == f.texture.shape, k
+        single = rlic_amd.convolve(f.texture, f.u, f.v, kernel=f.kernel, uv_mode=f.uv_mode,
+                                   boundaries=f.boundaries, iterations=f.iterations)
+        assert np.array_equal(out, single, equal_nan=True), k
+
+    def t(a):
+        return torch.from_numpy(np.ascontiguousarray(a)).to(gpu_device)
+
+    dev_fields = [Field(t(f.texture), t(f.u), t(f.v), t(f.kernel), uv_mode=f.uv_mode,
+                        boundaries=f.boundaries, iterations=f.iterations) for f in fields]
+    assert dev_fields[1].texture.numel() == 0
+    outs = convolve_batch(dev_fields)
+    torch.cuda.synchronize()
+    for k, (f, out) in enumerate(zip(fields, outs)):
+        single = rlic_amd.convolve(f.texture, f.u, f.v, kernel=f.kernel, uv_mode=f.uv_mode,
+                                   boundaries=f.boundaries, iterations=f.iterations)
+        assert np.array_equal(out.cpu().numpy(), single, equal_nan=True), k
+    # a batch of empty fields only
+    only = convolve_batch([dev_fields[1], dev_fields[3]])
+    assert [tuple(o.shape) for o in only] == [(0, 5), (5, 0)]
+
+
+@pytest.mark.gpu
+def test_uneven_batch_at_full_size(gpu_device):
+    """The batched uneven workload at full size: 512 float32 fields, sides
+    log-uniform in 128..2048 (aspect 1:4..4:1), odd kernel lengths 17..257
+    (triangular or boxcar), random periodic/closed axes and uv_mode, 1..10
+    iterations.  Every field equals its single call bit for bit; fields of
+    at most 256^2 pixels equal the oracle whole, one-pass larger fields on a
+    band of rows (the reference semantics, src/lib.rs:408-443)."""
+    import torch
+
+    from rlic_amd import _core
+    from rlic_amd.workloads import batch_uneven
+
+    fields = batch_uneven(512, seed=4, device=gpu_device)
+    sides = [s for f in fields for s in f["texture"].shape]
+    assert min(sides) >= 128 and max(sides) <= 2048
+    outs = convolve_batch(fields)
+    torch.cuda.synchronize()
+    checked = {"whole": 0, "band": 0}
+    for k, (f, out) in enumerate(zip(fields, outs)):
+        bs = rlic_amd._boundaries.BoundarySet.from_spec(f["boundaries"])
+        single = _core.convolve_f32(f["texture"], (f["u"], f["v"], f["uv_mode"]), f["kernel"],
+                                    (bs.x, bs.y), f["iterations"])
+        assert same_bits(out, single), f"field {k} differs from its single call"
+        small = f["texture"].numel() <= 256 * 256
+        if (small and checked["whole"] < 16) or (f["iterations"] == 1 and checked["band"] < 24):
+            how = assert_matches_oracle(out, f["texture"], f["u"], f["v"], kernel=f["kernel"],
+                                        uv_mode=f["uv_mode"], boundaries=f["boundaries"],
+                                        iterations=f["iterations"], what=f"field {k}")
+            checked[how] += 1
+    assert checked["whole"] >= 8 and checked["band"] >= 8, checked
 
 
 def test_batch_validation_runs_first():
@@ -207,8 +281,11 @@ def test_batch_and_serving_follow_each_arithmetic_profile(gpu_device, profile):
         fn = _core.convolve_f32 if f.texture.dtype == np.float32 else _core.convolve_f64
         single = fn(f.texture, (f.u, f.v, f.uv_mode), f.kernel, (bs.x, bs.y), f.iterations,
                     profile=profile)
-        assert np.array_equal(outs[k], single, equal_nan=True), k
-        assert np.array_equal(served[k].result, single, equal_nan=True), k
+        exp = oracle_convolve(f.texture, f.u, f.v, kernel=f.kernel, uv_mode=f.uv_mode,
+                              boundaries=f.boundaries, iterations=f.iterations, profile=profile)
+        assert np.array_equal(single, exp, equal_nan=True), k
+        assert np.array_equal(outs[k], exp, equal_nan=True), k
+        assert np.array_equal(served[k].result, exp, equal_nan=True), k
 
 
 def test_quick_structure_check_agrees_with_validate():

@@ -1,7 +1,8 @@
-"""Serving: requests batched at pass granularity give, each, the same bits
-as a single convolve call — alone, in a mix, arriving between steps, under
-a tight memory budget, and as follow-ups of kept fields (GPU); the budget
-arithmetic and the C ABI's argument checks (CPU)."""
+"""Serving: requests batched at pass granularity give, each, the oracle's
+bits (and those of a single convolve call) — alone, in a mix, arriving
+between steps, under a tight memory budget, as follow-ups of kept fields,
+after eviction and re-admission, and on the serving workload's distribution
+(GPU); the budget arithmetic and the C ABI's argument checks (CPU)."""
 
 from __future__ import annotations
 
@@ -11,6 +12,7 @@ import numpy as np
 import pytest
 
 import rlic_amd
+from oracle_check import assert_matches_oracle, oracle_convolve, same_bits, to_numpy
 from rlic_amd import _native
 
 
@@ -35,9 +37,17 @@ def _requests(rng, n, max_side=100):
 
 
 def _expected(r, iterations=None):
-    return rlic_amd.convolve(r["texture"], r["u"], r["v"], kernel=r["kernel"],
-                             uv_mode=r["uv_mode"], boundaries=r["boundaries"],
-                             iterations=r["iterations"] if iterations is None else iterations)
+    """The oracle's result for request `r` with `iterations` passes in all
+    (src/lib.rs:432-440 chains the passes: a follow-up of k passes after n
+    equals one call with n + k); the GPU single call must give the same
+    bits, and served results are compared with both."""
+    n = r["iterations"] if iterations is None else iterations
+    exp = oracle_convolve(r["texture"], r["u"], r["v"], kernel=r["kernel"],
+                          uv_mode=r["uv_mode"], boundaries=r["boundaries"], iterations=n)
+    single = rlic_amd.convolve(r["texture"], r["u"], r["v"], kernel=r["kernel"],
+                               uv_mode=r["uv_mode"], boundaries=r["boundaries"], iterations=n)
+    assert np.array_equal(to_numpy(single), exp, equal_nan=True), "single call vs oracle"
+    return exp
 
 
 def _submit(server, r, **kw):
@@ -162,7 +172,8 @@ def test_device_tensors_and_edge_cases(gpu_device):
             srv.submit(t(-tex), t(u), t(v), kernel=t(k))
         srv.run()
     assert isinstance(a.result, torch.Tensor) and a.result.device == torch.device(gpu_device)
-    exp = rlic_amd.convolve(tex, u, v, kernel=k, iterations=3)
+    exp = _expected(dict(texture=tex, u=u, v=v, kernel=k, uv_mode="velocity",
+                         boundaries="closed", iterations=3))
     assert np.array_equal(a.result.cpu().numpy(), exp)
     assert np.array_equal(b.result.cpu().numpy(), tex)
     assert c.result.shape == (0, 5)
@@ -244,3 +255,68 @@ def test_large_fields_replay_as_single_launches(gpu_device):
         assert np.array_equal(h.result, _expected(r), equal_nan=True), k
     for r, h in zip(reqs[-3:], more):
         assert np.array_equal(h.result, _expected(r, 5), equal_nan=True)
+
+
+@pytest.mark.gpu
+def test_serving_workload_distribution(gpu_device):
+    """The serving workload's distribution under its 64 GiB budget: fields
+    of sides 128..2048 (aspect 1:4..4:1), float32 and float64 alternating,
+    odd kernel lengths 17..257, random axes and uv_mode, 1..20 passes;
+    requests arrive between steps (Poisson arrivals, counted in steps), half
+    of them keep their field and a quarter of the arrivals follow up a
+    finished kept request.  Every result equals the single call with the
+    request's total passes, bit for bit; small fields equal the oracle
+    whole, one-pass large fields on a band of rows."""
+    import torch
+
+    from rlic_amd.serve import Server
+    from rlic_amd.workloads import serving_pool, serving_request
+
+    pool = serving_pool(24, seed=5, device=gpu_device)
+    rng = np.random.default_rng(55)
+    n_req = 40
+    # arrival step of each request: Poisson process of 1.5 arrivals per step
+    arrive_at = np.floor(np.cumsum(rng.exponential(1 / 1.5, size=n_req))).astype(int)
+    plans = [(serving_request(rng, pool), rng.random() < 0.25, rng.random() < 0.5,
+              int(rng.integers(1, 21))) for _ in range(n_req)]
+    done_reqs = []  # (request, spec dict, total passes)
+    kept = []
+    with Server(64 << 30, device=gpu_device) as srv:
+        k = 0
+        step = 0
+        while k < n_req or srv.queued or srv.active:
+            while k < n_req and arrive_at[k] <= step:
+                r, follow, keep, more = plans[k]
+                ready = [(q, spec) for q, spec in kept if q.state == "done"]
+                if follow and ready:
+                    base, spec = ready[int(rng.integers(len(ready)))]
+                    kept.remove((base, spec))
+                    q = srv.follow_up(base, more)
+                    done_reqs.append((q, spec))
+                    kept.append((q, spec))
+                else:
+                    q = _submit(srv, r, keep=keep)
+                    done_reqs.append((q, r))
+                    if keep:
+                        kept.append((q, r))
+                k += 1
+            srv.step()
+            step += 1
+        assert srv.queued == 0 and srv.active == 0
+    assert sum(q.base is not None for q, _ in done_reqs) >= 3
+    assert {q.texture.dtype for q, _ in done_reqs} == {torch.float32, torch.float64}
+    checked = {"whole": 0, "band": 0, None: 0}
+    for idx, (q, spec) in enumerate(done_reqs):
+        assert q.state == "done", idx
+        n = q.total_iterations
+        single = rlic_amd.convolve(spec["texture"], spec["u"], spec["v"], kernel=spec["kernel"],
+                                   uv_mode=spec["uv_mode"], boundaries=spec["boundaries"],
+                                   iterations=n)
+        assert same_bits(q.result, single), f"request {idx} differs from its single call"
+        if spec["texture"].numel() <= 256 * 256 or n == 1:
+            how = assert_matches_oracle(q.result, spec["texture"], spec["u"], spec["v"],
+                                        kernel=spec["kernel"], uv_mode=spec["uv_mode"],
+                                        boundaries=spec["boundaries"], iterations=n,
+                                        what=f"request {idx}")
+            checked[how] += 1
+    assert checked["whole"] + checked["band"] >= 4, checked
