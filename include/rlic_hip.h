@@ -98,6 +98,44 @@ int rlic_convolve_f64(const double* texture, int64_t tex_sy, int64_t tex_sx,
                       double* out,
                       int location, int profile, void* stream);
 
+/* Allocation-free variant of rlic_convolve_f32 / _f64 for device arrays.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, checks and results as
+ * rlic_convolve_* with location = RLIC_DEVICE (enqueued on `stream`;
+ * returns without waiting).  The call's scratch (padded weights, guarded
+ * cells, ping-pong buffer, move record) lives in one fixed device array of
+ * RLIC_NOALLOC_SCRATCH_BYTES that each GPU holds from its first such call
+ * until the process ends.  A call whose weights, cells and ping-pong buffer
+ * do not fit fails with RLIC_ERR_INVALID_ARGUMENT before any work; one whose
+ * move record does not also fit traces every pass instead of replaying
+ * (same bits, more work).  No hipMalloc, hipFree, events or graphs.
+ * Every call on a GPU shares that GPU's array and leaves its contents
+ * there: calls on one stream are ordered, but calls that overlap on one GPU
+ * (other streams or other host threads) race, and the caller must
+ * serialize them. */
+#define RLIC_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int rlic_convolve_noalloc_f32(const float* texture, int64_t tex_sy, int64_t tex_sx,
+                              const float* u, int64_t u_sy, int64_t u_sx,
+                              const float* v, int64_t v_sy, int64_t v_sx,
+                              int64_t ny, int64_t nx,
+                              const float* kernel, int64_t kernel_len,
+                              int uv_mode,
+                              int x_left, int x_right, int y_left, int y_right,
+                              int64_t iterations,
+                              float* out,
+                              int profile, void* stream);
+int rlic_convolve_noalloc_f64(const double* texture, int64_t tex_sy, int64_t tex_sx,
+                              const double* u, int64_t u_sy, int64_t u_sx,
+                              const double* v, int64_t v_sy, int64_t v_sx,
+                              int64_t ny, int64_t nx,
+                              const double* kernel, int64_t kernel_len,
+                              int uv_mode,
+                              int x_left, int x_right, int y_left, int y_right,
+                              int64_t iterations,
+                              double* out,
+                              int profile, void* stream);
+
 /* One field of a batched call: the arguments of one rlic_convolve_* call
  * with DEVICE pointers (see rlic_convolve_batch_*). */
 typedef struct {

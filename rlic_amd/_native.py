@@ -90,6 +90,9 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "rlic_convolve_batch_f64": ([ctypes.POINTER(FieldF64), _i64, _int, _vp], _int),
     "rlic_convolve_f32": (_CONVOLVE_ARGTYPES, _int),
     "rlic_convolve_f64": (_CONVOLVE_ARGTYPES, _int),
+    # the allocation-free variants take no `location` (device arrays only)
+    "rlic_convolve_noalloc_f32": (_CONVOLVE_ARGTYPES[:20] + _CONVOLVE_ARGTYPES[21:], _int),
+    "rlic_convolve_noalloc_f64": (_CONVOLVE_ARGTYPES[:20] + _CONVOLVE_ARGTYPES[21:], _int),
     "rlic_last_error": ([], ctypes.c_char_p),
     "rlic_trace_stats_read": ([_i64p, _i64p], _int),
     "rlic_autotune_read": ([ctypes.c_char_p, _i64], _i64),

@@ -76,6 +76,10 @@ std::map<std::string, std::pair<int64_t, double>> g_prof_done;  // name -> (coun
 std::mutex g_stats_mu;
 int64_t g_stats_useful = 0, g_stats_executed = 0;
 
+// Set while this thread runs an allocation-free call: no events are
+// created for it (hipEventCreate may allocate).
+thread_local bool t_noalloc = false;
+
 struct LaunchTimer {
   const char* name;
   hipStream_t stream;
@@ -83,6 +87,7 @@ struct LaunchTimer {
   bool on = false;
   int64_t units = 1;
   LaunchTimer(const char* n, hipStream_t s, int64_t u = 1) : name(n), stream(s), units(u) {
+    if (t_noalloc) return;
     {
       std::lock_guard<std::mutex> lock(g_prof_mu);
       on = g_prof_on;
@@ -449,11 +454,45 @@ bool replay_graph(int device, const ReplayPlan& plan, const T* kernel, int32_t k
   return true;
 }
 
+// One call's scratch block: padded weights | cells | ping-pong buffer | move
+// record (the cells a guarded array: a ring of guard cells around the
+// image).  `codes` is where the move record starts; `codes + code_bytes`
+// is the whole block with the record, `codes` without it.
+struct ScratchLayout {
+  size_t cells, tmp, codes, code_bytes;
+};
+
+template <typename T>
+ScratchLayout scratch_layout(int32_t ny, int32_t nx, int32_t klen, bool need_tmp) {
+  auto up = [](size_t x) { return (x + 255) & ~size_t{255}; };
+  const size_t n = static_cast<size_t>(ny) * static_cast<size_t>(nx);
+  const size_t ncells = static_cast<size_t>(rlic::guarded_cells(ny, nx));
+  ScratchLayout l;
+  l.cells = up(sizeof(T) * static_cast<size_t>(klen + 2 * rlic::kernel_pad()));
+  l.tmp = l.cells + up(sizeof(rlic::Cell<T>) * ncells);
+  l.codes = l.tmp + (need_tmp ? up(sizeof(T) * n) : 0);
+  // code_chunks(klen) 16-byte chunks per pixel
+  l.code_bytes = sizeof(uint4) * static_cast<size_t>(rlic::code_chunks(klen)) * n;
+  return l;
+}
+
+// A caller-provided scratch region (the allocation-free entry points):
+// `bytes` at `ptr`, device memory of the current device.
+struct FixedScratch {
+  void* ptr;
+  size_t bytes;
+};
+
+// `fixed`: take the call's scratch from that region and allocate nothing
+// (no move record when it does not fit: every pass is traced; no trace
+// statistics, autotuning, graph capture or launch timing, which create HIP
+// objects); nullptr: from the per-stream cache or the stream-ordered pool.
 template <typename T>
 int run_on_device(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t usy, int64_t usx,
                   const T* v, int64_t vsy, int64_t vsx, int32_t ny, int32_t nx,
                   const T* kernel, int32_t klen, int uv_mode, const rlic::FieldGeom& g,
-                  int64_t iterations, T* out, int profile, hipStream_t stream) {
+                  int64_t iterations, T* out, int profile, hipStream_t stream,
+                  const FixedScratch* fixed = nullptr) {
   if (iterations == 0) {
     LaunchTimer timer("lic_copy2d", stream);
     RLIC_HIP_TRY(rlic::launch_copy2d<T>(tex, tsy, tsx, ny, nx, out, stream), "copy");
@@ -466,35 +505,38 @@ int run_on_device(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
     dev = -1;
   }
   const int32_t pad = rlic::kernel_pad();
-  const bool stats = env_int("RLIC_TRACE_STATS", 0) != 0;
+  const bool stats = !fixed && env_int("RLIC_TRACE_STATS", 0) != 0;
   const bool need_tmp = iterations > 1 || stats;
-  // Move record: code_chunks(klen) 16-byte chunks per pixel.
-  const size_t code_bytes =
-      sizeof(uint4) * static_cast<size_t>(rlic::code_chunks(klen)) * static_cast<size_t>(n);
+  const ScratchLayout lay = scratch_layout<T>(ny, nx, klen, need_tmp);
+  const size_t code_bytes = lay.code_bytes;
   bool replay = need_tmp && env_int("RLIC_REPLAY", 1) != 0;
-  if (replay && code_bytes > (size_t{256} << 20)) {
+  if (!fixed && replay && code_bytes > (size_t{256} << 20)) {
     // a large record must leave half of the free memory; small ones skip
     // the (slow) query and fall back below if the allocation fails
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
     replay = code_bytes <= free_b / 2;
   }
-  // one scratch block: padded weights | cells | ping-pong buffer | move record
-  // (the cells a guarded array: a ring of guard cells around the image)
-  auto up = [](size_t x) { return (x + 255) & ~size_t{255}; };
-  const size_t ncells = static_cast<size_t>(rlic::guarded_cells(ny, nx));
-  const size_t off_cells = up(sizeof(T) * static_cast<size_t>(klen + 2 * pad));
-  const size_t off_tmp = off_cells + up(sizeof(rlic::Cell<T>) * ncells);
-  const size_t off_codes = off_tmp + (need_tmp ? up(sizeof(T) * static_cast<size_t>(n)) : 0);
+  const size_t off_cells = lay.cells, off_tmp = lay.tmp, off_codes = lay.codes;
   rlic::CallScratch scratch;
-  hipError_t ae = scratch.get(off_codes + (replay ? code_bytes : 0), stream);
-  if (ae != hipSuccess && replay) {  // no room for the record: trace every pass
-    (void)hipGetLastError();
-    replay = false;
-    ae = scratch.get(off_codes, stream);
+  char* base;
+  if (fixed) {
+    if (off_codes > fixed->bytes)
+      return set_error(RLIC_ERR_INVALID_ARGUMENT,
+                       "the call needs %zu bytes of scratch; the fixed scratch holds %zu",
+                       off_codes, fixed->bytes);
+    if (replay && off_codes + code_bytes > fixed->bytes) replay = false;  // trace every pass
+    base = static_cast<char*>(fixed->ptr);
+  } else {
+    hipError_t ae = scratch.get(off_codes + (replay ? code_bytes : 0), stream);
+    if (ae != hipSuccess && replay) {  // no room for the record: trace every pass
+      (void)hipGetLastError();
+      replay = false;
+      ae = scratch.get(off_codes, stream);
+    }
+    RLIC_HIP_TRY(ae, "allocating call scratch");
+    base = static_cast<char*>(scratch.ptr());
   }
-  RLIC_HIP_TRY(ae, "allocating call scratch");
-  char* base = static_cast<char*>(scratch.ptr());
   T* kpad = reinterpret_cast<T*>(base);
   auto* c = reinterpret_cast<rlic::Cell<T>*>(base + off_cells);
   T* tmp = need_tmp ? reinterpret_cast<T*>(base + off_tmp) : nullptr;
@@ -553,13 +595,15 @@ int run_on_device(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
     ReplayPlan plan = default_plan<T>(klen, n);
     int64_t first = 2;
     if (env_int("RLIC_AUTOTUNE", 1) != 0 && iterations >= 3 && n >= (int64_t{1} << 18)) {
+      // an allocation-free call uses a tuned plan when the table has one,
+      // and never tunes (the timing events are HIP objects)
       const TuneKey key{static_cast<int32_t>(sizeof(T)), ny, nx, klen, profile};
       std::lock_guard<std::mutex> lock(g_tuner.mu);
       tuner_load_locked();
       auto hit = g_tuner.table.find(key);
       if (hit != g_tuner.table.end()) {
         if (plan_usable<T>(hit->second.plan, klen, n)) plan = hit->second.plan;
-      } else {
+      } else if (!fixed) {
         const std::vector<ReplayPlan> cands = replay_candidates<T>(klen, n);
         TuneEntry best{plan, -1.f};
         RLIC_HIP_TRY(tune_replay<T>(cands, a, b, kernel, klen, g, profile, codes, stream, best),
@@ -571,7 +615,7 @@ int run_on_device(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
     }
     // passes first..n: direct launches, or one captured graph with
     // RLIC_GRAPH=1 (measured no faster on MI355X: kept as an option)
-    if (first == 2 && env_int("RLIC_GRAPH", 0) != 0 &&
+    if (first == 2 && !fixed && env_int("RLIC_GRAPH", 0) != 0 &&
         replay_graph<T>(dev, plan, kernel, klen, g, profile, a, b, codes, iterations, stream))
       return RLIC_OK;
     for (int64_t it = first; it <= iterations; ++it) {
@@ -835,6 +879,55 @@ int64_t server_field_bytes(int64_t ny, int64_t nx, int64_t klen, int keep, int64
       rlic::Server::field_bytes<T>(ny, nx, klen, keep != 0 || iterations > 1));
 }
 
+// Allocation-free convolve: the same checks and computation as
+// convolve_entry with device arrays, its scratch in the fixed device array
+// (see rlic_convolve_noalloc_* below).
+int noalloc_scratch_ptr(void** ptr);
+
+template <typename T>
+int convolve_noalloc_entry(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t usy,
+                           int64_t usx, const T* v, int64_t vsy, int64_t vsx, int64_t ny,
+                           int64_t nx, const T* kernel, int64_t klen, int uv_mode, int xl, int xr,
+                           int yl, int yr, int64_t iterations, T* out, int profile,
+                           void* stream_ptr) {
+  t_last_error.clear();
+  if (profile < 0 || profile > 3)
+    return set_error(RLIC_ERR_INVALID_ARGUMENT, "unknown arithmetic profile %d", profile);
+  if (const int rc = validate<T>(tex, u, v, ny, nx, kernel, klen, uv_mode, xl, xr, yl, yr,
+                                 iterations, out);
+      rc != RLIC_OK)
+    return rc;
+  if (ny == 0 || nx == 0) return RLIC_OK;  // nothing to compute
+  if (iterations > 0) {
+    // weights, cells and ping-pong buffer must fit; the move record may not
+    const size_t need = scratch_layout<T>(static_cast<int32_t>(ny), static_cast<int32_t>(nx),
+                                          static_cast<int32_t>(klen), iterations > 1)
+                            .codes;
+    if (need > static_cast<size_t>(RLIC_NOALLOC_SCRATCH_BYTES))
+      return set_error(RLIC_ERR_INVALID_ARGUMENT,
+                       "an image of %lld x %lld pixels with a kernel of %lld weights needs %zu "
+                       "bytes of scratch, more than the %lld bytes of the allocation-free "
+                       "entry point (use rlic_convolve_f32/_f64)",
+                       static_cast<long long>(ny), static_cast<long long>(nx),
+                       static_cast<long long>(klen), need,
+                       static_cast<long long>(RLIC_NOALLOC_SCRATCH_BYTES));
+  }
+  void* ptr = nullptr;
+  if (iterations > 0) {
+    if (const int rc = noalloc_scratch_ptr(&ptr); rc != RLIC_OK) return rc;
+  }
+  struct Guard {  // no HIP objects are created for this call's launches
+    Guard() { t_noalloc = true; }
+    ~Guard() { t_noalloc = false; }
+  } guard;
+  const FixedScratch fixed{ptr, static_cast<size_t>(RLIC_NOALLOC_SCRATCH_BYTES)};
+  const rlic::FieldGeom g{static_cast<int32_t>(ny), static_cast<int32_t>(nx), xl, xr, yl, yr};
+  return run_on_device<T>(tex, tsy, tsx, u, usy, usx, v, vsy, vsx, static_cast<int32_t>(ny),
+                          static_cast<int32_t>(nx), kernel, static_cast<int32_t>(klen), uv_mode,
+                          g, iterations, out, profile, static_cast<hipStream_t>(stream_ptr),
+                          &fixed);
+}
+
 template <typename T>
 int server_read(rlic::Server* server, int64_t handle, T* out) {
   t_last_error.clear();
@@ -904,6 +997,39 @@ int fill_modes(T* out, int64_t ny, int64_t nx, int32_t n_modes, const double* am
 
 }  // namespace
 
+// The allocation-free entry points' scratch: one fixed array per device,
+// held until the process ends.  In host code `scratch` names a host-side
+// stand-in, not device memory: host code uses only the device address that
+// hipGetSymbolAddress returns for the current device, taken once per
+// device on that device's first allocation-free call and kept.
+static __device__ __align__(256) unsigned char scratch[RLIC_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 256;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kMaxDevices] = {};
+
+// The current device's address of `scratch` (looked up on its first use).
+int noalloc_scratch_ptr(void** ptr) {
+  int dev = -1;
+  RLIC_HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices)
+    return set_error(RLIC_ERR_INVALID_ARGUMENT, "device index %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_error(e, "locating the fixed scratch array");
+    if (!p) return set_error(RLIC_ERR_HIP, "locating the fixed scratch array: null address");
+    g_noalloc_ptr[dev] = p;
+  }
+  *ptr = g_noalloc_ptr[dev];
+  return RLIC_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int rlic_convolve_f32(const float* texture, int64_t tex_sy, int64_t tex_sx, const float* u,
@@ -925,6 +1051,28 @@ int rlic_convolve_f64(const double* texture, int64_t tex_sy, int64_t tex_sx, con
   return convolve_entry<double>(texture, tex_sy, tex_sx, u, u_sy, u_sx, v, v_sy, v_sx, ny, nx,
                                 kernel, kernel_len, uv_mode, x_left, x_right, y_left, y_right,
                                 iterations, out, location, profile, stream);
+}
+
+int rlic_convolve_noalloc_f32(const float* texture, int64_t tex_sy, int64_t tex_sx,
+                              const float* u, int64_t u_sy, int64_t u_sx, const float* v,
+                              int64_t v_sy, int64_t v_sx, int64_t ny, int64_t nx,
+                              const float* kernel, int64_t kernel_len, int uv_mode, int x_left,
+                              int x_right, int y_left, int y_right, int64_t iterations,
+                              float* out, int profile, void* stream) {
+  return convolve_noalloc_entry<float>(texture, tex_sy, tex_sx, u, u_sy, u_sx, v, v_sy, v_sx,
+                                       ny, nx, kernel, kernel_len, uv_mode, x_left, x_right,
+                                       y_left, y_right, iterations, out, profile, stream);
+}
+
+int rlic_convolve_noalloc_f64(const double* texture, int64_t tex_sy, int64_t tex_sx,
+                              const double* u, int64_t u_sy, int64_t u_sx, const double* v,
+                              int64_t v_sy, int64_t v_sx, int64_t ny, int64_t nx,
+                              const double* kernel, int64_t kernel_len, int uv_mode, int x_left,
+                              int x_right, int y_left, int y_right, int64_t iterations,
+                              double* out, int profile, void* stream) {
+  return convolve_noalloc_entry<double>(texture, tex_sy, tex_sx, u, u_sy, u_sx, v, v_sy, v_sx,
+                                        ny, nx, kernel, kernel_len, uv_mode, x_left, x_right,
+                                        y_left, y_right, iterations, out, profile, stream);
 }
 
 int rlic_convolve_batch_f32(const rlic_field_f32* fields, int64_t n_fields, int profile,
