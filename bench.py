@@ -9,16 +9,27 @@ are already resident in HBM as device tensors.
 
 Prints ONE JSON line:
   value     Gpixel-steps/s = H*W*(L-1)*iterations per step / step time
-  roofline  for the kernel with the most total time in the timed region:
-            algorithmic bytes per launch (the reference's gathers per pass:
-            (3*(L-1)+2)*sizeof(T) per pixel) / average launch time measured
-            with HIP events on the launch stream, against 8 TB/s HBM;
-            `traffic` = measured HBM bytes per launch from the committed
-            rocprofv3 counter summary (null when absent)
+  roofline  for the kernel with the most total time in the timed region
+            (lic_trace): it is bound by VALU instruction issue, so `achieved`
+            is its VALU lane-instructions per launch (wave64 instructions from
+            the committed rocprofv3 counter summary, x 64) / its average
+            launch time measured live with HIP events on the launch stream,
+            against MI355X's FP32 vector peak expressed in lane-instructions
+            (157.3 TFLOP/s / 2 flops per FMA = 78.6 T/s: 128 lanes per CU per
+            clock x 256 CUs x 2.4 GHz); `hbm` = counter-measured HBM bytes per
+            launch / launch time against 8 TB/s; `logical_gather_gbs` = the
+            reference algorithm's gathers ((3*(L-1)+2)*sizeof(T) per pixel)
+            per launch / launch time, served mostly by caches and LDS.
+  extras    the other configurations, each timed here: the README example
+            (256^2 float64), the polarization fields (16384^2 float32 and
+            8192^2 float64, generated on the device), the 512-field uneven
+            batch (batched vs per-call speedup) and serving (Poisson arrivals:
+            throughput, p50/p99 latency).  --no-extras skips them.
   cpu_baseline  the CPU oracle (a restatement of the reference's
             single-threaded algorithm) on a bounded band of rows of pass 1.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+                       [--no-extras]
 """
 
 from __future__ import annotations
@@ -37,6 +48,9 @@ if str(ROOT) not in sys.path:
     sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# MI355X FP32 vector peak, 157.3 TFLOP/s, in VALU lane-instructions per second:
+# one FMA lane-instruction is 2 flops, i.e. 128 lanes x 256 CUs x 2.4 GHz
+VALU_PEAK_TLANE = 157.3 / 2
 
 
 def log(msg: str) -> None:
@@ -51,6 +65,20 @@ def traffic_from_profiles(kernel: str, workload: str) -> float | None:
     try:
         data = json.loads(path.read_text())
         return float(data["hbm_bytes_per_launch"])
+    except (ValueError, KeyError):
+        return None
+
+
+def valu_from_profiles(kernel: str, workload: str) -> dict | None:
+    """Wave64 VALU instructions and waves per launch from the committed
+    counter summary (SQ_INSTS_VALU, SQ_WAVES), if any."""
+    path = ROOT / "profiles" / f"valu_{kernel}_{workload}.json"
+    if not path.is_file():
+        return None
+    try:
+        data = json.loads(path.read_text())
+        return {"insts": float(data["valu_insts_per_launch"]),
+                "waves": float(data["waves_per_launch"])}
     except (ValueError, KeyError):
         return None
 
@@ -88,7 +116,7 @@ def cpu_baseline(wl, rows: int) -> dict:
     cpu.convolve(wl.texture, wl.u, wl.v, kernel=wl.kernel, uv_mode=wl.uv_mode,
                  boundaries=wl.boundaries, iterations=1, rows=(r0, r1), threads=1)
     dt = time.perf_counter() - t0
-    steps = (r1 - r0) * nx * (wl.kernel.size - 1)
+    steps = (r1 - r0) * nx * (wl.kernel_len - 1)
     # the same band with OpenMP over rows on the host's cores (at most 16:
     # one GPU's share of the host), a parallel extension of the reference's
     # single-threaded loop
@@ -110,16 +138,18 @@ def cpu_baseline(wl, rows: int) -> dict:
     }
 
 
-def bench_batch(args, dev) -> int:
-    """512 uneven fields (sides 128..2048, or 32..256 for batch512small): one
-    persistent batched launch vs a per-call loop."""
+def batch_measure(dev, steps: int, warmup: int, small: bool = False) -> dict:
+    """512 uneven fields (sides 128..2048, or 32..256 with `small`): one
+    batched call vs a per-call loop of plain GPU calls, both on
+    device-resident inputs, timed from a synchronized start to a
+    synchronized end over `steps` repetitions after `warmup`."""
     import torch
 
     from rlic_amd import _core, _native
+    from rlic_amd._boundaries import BoundarySet
     from rlic_amd.batch import convolve_batch
     from rlic_amd.workloads import batch_uneven, pixel_steps
 
-    small = args.workload == "batch512small"
     log(f"bench: generating 512 uneven fields on the device (sides "
         f"{'32..256' if small else '128..2048'})")
     fields = (batch_uneven(512, seed=4, device=dev, min_side=32, max_side=256) if small
@@ -127,7 +157,7 @@ def bench_batch(args, dev) -> int:
     steps_total = pixel_steps(fields)
     bpairs = []
     for f in fields:
-        bs = __import__("rlic_amd")._boundaries.BoundarySet.from_spec(f["boundaries"])
+        bs = BoundarySet.from_spec(f["boundaries"])
         bpairs.append((bs.x, bs.y))
 
     def per_call():
@@ -140,14 +170,14 @@ def bench_batch(args, dev) -> int:
 
     timings = {}
     for name, fn in (("per_call", per_call), ("batched", batched)):
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             fn()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             fn()
         torch.cuda.synchronize()
-        timings[name] = (time.perf_counter() - t0) / args.steps
+        timings[name] = (time.perf_counter() - t0) / steps
         log(f"bench: {name} {timings[name] * 1e3:.2f} ms per batch")
     # kernel times from one more, untimed batched call: the launch profiler's
     # events add host work, so it stays off while timing
@@ -157,28 +187,43 @@ def bench_batch(args, dev) -> int:
     torch.cuda.synchronize()
     prof = _native.profiler_read()
     _native.profiler_enable(False)
-    value = steps_total / timings["batched"] / 1e9
-    per_call_value = steps_total / timings["per_call"] / 1e9
+    del fields
+    torch.cuda.empty_cache()
+    return {
+        "workload": "batch_uneven_512_f32" + ("_small" if small else ""),
+        "fields": 512, "sides": "32..256" if small else "128..2048",
+        "pixel_steps_per_batch": steps_total,
+        "batched": {"value": round(steps_total / timings["batched"] / 1e9, 3),
+                    "ms_per_batch": round(timings["batched"] * 1e3, 3)},
+        "per_call": {"value": round(steps_total / timings["per_call"] / 1e9, 3),
+                     "ms_per_batch": round(timings["per_call"] * 1e3, 3)},
+        "batched_vs_per_call_speedup": round(timings["per_call"] / timings["batched"], 3),
+        "kernel_ms_per_batch": {k: round(v["total_ms"], 3) for k, v in prof.items()},
+        "steps": steps, "warmup": warmup,
+    }
+
+
+def bench_batch(args, dev) -> int:
+    """The uneven batch as the benchmark's one line."""
+    m = batch_measure(dev, args.steps, args.warmup, small=args.workload == "batch512small")
     line = {
         "metric": "Gpixel-steps/s (batched uneven fields)",
-        "value": round(value, 3),
+        "value": m["batched"]["value"],
         "unit": "Gpixel-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(timings["batched"] * 1e3, 3),
+        "ms_per_step": m["batched"]["ms_per_batch"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "batch_uneven_512_f32" + ("_small" if small else ""),
-                   "fields": len(fields), "sides": "32..256" if small else "128..2048",
-                   "pixel_steps_per_step": steps_total},
-        "per_call": {"value": round(per_call_value, 3),
-                     "ms_per_step": round(timings["per_call"] * 1e3, 3)},
-        "batched_vs_per_call_speedup": round(timings["per_call"] / timings["batched"], 3),
-        "kernel_ms_per_step": {k: round(v["total_ms"], 3) for k, v in prof.items()},
+        "config": {"workload": m["workload"], "fields": m["fields"], "sides": m["sides"],
+                   "pixel_steps_per_step": m["pixel_steps_per_batch"]},
+        "per_call": m["per_call"],
+        "batched_vs_per_call_speedup": m["batched_vs_per_call_speedup"],
+        "kernel_ms_per_step": m["kernel_ms_per_batch"],
         "roofline": None,
         "cpu_baseline": None,
     }
@@ -186,8 +231,12 @@ def bench_batch(args, dev) -> int:
     return 0
 
 
-def bench_serve(args, dev) -> int:
-    """Serving: Poisson arrivals of convolve requests, continuous batching."""
+def serve_measure(dev, burst: int, n_requests: int, pool_fields: int = 96) -> dict:
+    """Serving: Poisson arrivals of convolve requests on a continuous-batching
+    server (64 GiB budget, fields of the uneven batch's distribution, float32
+    and float64, 1..20 passes, a quarter of the arrivals following up a
+    finished kept request).  The arrival rate is 70% of the capacity
+    measured on a burst of `burst` requests arriving at once."""
     import torch
 
     from rlic_amd.serve import Server
@@ -195,7 +244,7 @@ def bench_serve(args, dev) -> int:
 
     budget = 64 << 30
     log("bench: generating the serving field pool on the device")
-    pool = serving_pool(96, seed=5, device=dev)
+    pool = serving_pool(pool_fields, seed=5, device=dev)
     rng = np.random.default_rng(5)
 
     def submit(srv, r, arrival=None, keep=False):
@@ -208,10 +257,10 @@ def bench_serve(args, dev) -> int:
         for r in [serving_request(rng, pool) for _ in range(8)]:  # warm-up
             submit(srv, r)
         srv.run()
-        burst = [serving_request(rng, pool) for _ in range(args.serve_burst)]
+        reqs_in = [serving_request(rng, pool) for _ in range(burst)]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        reqs = [submit(srv, r) for r in burst]
+        reqs = [submit(srv, r) for r in reqs_in]
         srv.run()
         torch.cuda.synchronize()
         cap_s = time.perf_counter() - t0
@@ -225,7 +274,7 @@ def bench_serve(args, dev) -> int:
     # Poisson arrivals injected in real time by this thread while a
     # background thread steps the server; a quarter of the arrivals follow
     # up a finished kept request
-    n = args.serve_requests
+    n = n_requests
     gaps = rng.exponential(1.0 / lam, size=n)
     offsets = np.cumsum(gaps)
     plans = [(serving_request(rng, pool), rng.random() < 0.25, rng.random() < 0.5,
@@ -260,34 +309,119 @@ def bench_serve(args, dev) -> int:
     lat = np.array([q.latency for q in done]) * 1e3
     work = sum(q.pixel_steps for q in done)
     span = t_end - t_start
-    value = work / span / 1e9
+    del pool
+    torch.cuda.empty_cache()
+    return {
+        "workload": "serving_poisson", "requests": n, "completed": len(done),
+        "value": round(work / span / 1e9, 3), "unit": "Gpixel-steps/s",
+        "span_s": round(span, 3), "server_steps": steps_run,
+        "rate_per_s": round(lam, 2), "budget_bytes": budget,
+        "capacity_gpixel_steps_per_s": round(capacity / 1e9, 3),
+        "followups": sum(q.base is not None for q in done),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
+                       "p99": round(float(np.percentile(lat, 99)), 2),
+                       "max": round(float(lat.max()), 2)},
+        "evictions": evictions, "readmissions": readmissions,
+    }
+
+
+def bench_serve(args, dev) -> int:
+    """Serving as the benchmark's one line."""
+    m = serve_measure(dev, args.serve_burst, args.serve_requests)
     line = {
         "metric": "Gpixel-steps/s (serving, Poisson arrivals at 70% load)",
-        "value": round(value, 3),
+        "value": m["value"],
         "unit": "Gpixel-steps/s",
         "n_gpus": 1,
-        "steps": steps_run,
+        "steps": m["server_steps"],
         "warmup": 8,
-        "ms_per_step": round(span / max(steps_run, 1) * 1e3, 3),
+        "ms_per_step": round(m["span_s"] / max(m["server_steps"], 1) * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": "serving_poisson", "requests": n,
-                   "rate_per_s": round(lam, 2), "budget_bytes": budget,
-                   "capacity_gpixel_steps_per_s": round(capacity / 1e9, 3),
-                   "followups": sum(q.base is not None for q in done)},
-        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
-                       "p99": round(float(np.percentile(lat, 99)), 2),
-                       "max": round(float(lat.max()), 2)},
-        "evictions": evictions,
-        "readmissions": readmissions,
+        "config": {"workload": m["workload"], "requests": m["requests"],
+                   "rate_per_s": m["rate_per_s"], "budget_bytes": m["budget_bytes"],
+                   "capacity_gpixel_steps_per_s": m["capacity_gpixel_steps_per_s"],
+                   "followups": m["followups"]},
+        "latency_ms": m["latency_ms"],
+        "evictions": m["evictions"],
+        "readmissions": m["readmissions"],
         "roofline": None,
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def single_call_measure(wl, dev, steps: int, warmup: int) -> dict:
+    """One configuration as repeated convolve calls on device-resident
+    inputs: ms per call and Gpixel-steps/s (synchronized start and end),
+    and per-kernel times from the launch profiler in a separate, untimed
+    call."""
+    import torch
+
+    from rlic_amd import _core, _native
+    from rlic_amd._boundaries import BoundarySet
+
+    def dev_tensor(a):
+        if hasattr(a, "data_ptr"):
+            return a
+        return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+    t = [dev_tensor(a) for a in (wl.texture, wl.u, wl.v, wl.kernel)]
+    fn = _core.convolve_f32 if t[0].dtype == torch.float32 else _core.convolve_f64
+    bs = BoundarySet.from_spec(wl.boundaries)
+
+    def step():
+        return fn(t[0], (t[1], t[2], wl.uv_mode), t[3], (bs.x, bs.y), wl.iterations)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    _native.profiler_enable(True)
+    _native.profiler_read()
+    step()
+    torch.cuda.synchronize()
+    prof = _native.profiler_read()
+    _native.profiler_enable(False)
+    ny, nx = wl.shape
+    return {"workload": wl.name, "shape": [ny, nx], "kernel_len": wl.kernel_len,
+            "iterations": wl.iterations, "uv_mode": wl.uv_mode,
+            "ms_per_call": round(ms, 4),
+            "value": round(wl.pixel_steps / (ms / 1e3) / 1e9, 3), "unit": "Gpixel-steps/s",
+            "kernel_ms_per_call": {k: round(v["total_ms"], 4) for k, v in prof.items()},
+            "steps": steps, "warmup": warmup,
+            "inputs": wl.meta.get("generated_on", "host") + "-generated"}
+
+
+def extras_measure(dev, args) -> dict:
+    """The other configurations, each on its own inputs (freed after)."""
+    import torch
+
+    from rlic_amd.workloads import polarization_field_device, readme_example
+
+    out = {}
+    log("bench: extra readme_example (256^2 float64, L=65)")
+    out["readme_256_f64"] = single_call_measure(readme_example(), dev, steps=50, warmup=5)
+    for key, n, dtype in (("polarization_16384_f32", 16384, np.float32),
+                          ("polarization_8192_f64", 8192, np.float64)):
+        log(f"bench: extra {key} (inputs generated on the device)")
+        wl = polarization_field_device(n, dtype, device=dev)
+        out[key] = single_call_measure(wl, dev, steps=3, warmup=1)
+        del wl
+        torch.cuda.empty_cache()
+    log("bench: extra batch_uneven_512 (batched vs per-call)")
+    out["batch_uneven_512"] = batch_measure(dev, steps=3, warmup=1)
+    log("bench: extra serving")
+    out["serving_poisson"] = serve_measure(dev, args.serve_burst, args.serve_requests)
+    return out
 
 
 def main() -> int:
@@ -300,8 +434,10 @@ def main() -> int:
                     help="rows of pass 1 timed on the CPU oracle (0 = skip)")
     ap.add_argument("--serve-burst", type=int, default=96,
                     help="serving: requests of the capacity burst")
-    ap.add_argument("--serve-requests", type=int, default=1500,
+    ap.add_argument("--serve-requests", type=int, default=800,
                     help="serving: Poisson-arrival requests")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="time only the main workload (no other configurations)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="N",
                     help="first run the library's copy kernel N times on a 1 GiB array "
                          "(known HBM bytes, for calibrating FETCH_SIZE/WRITE_SIZE counters)")
@@ -373,20 +509,42 @@ def main() -> int:
     if dominant:
         avg_s = prof[dominant]["total_ms"] / prof[dominant]["count"] / 1e3
         b_alg = wl.algorithmic_bytes_per_pass()
-        achieved = b_alg / avg_s / 1e9
-        roofline = {
-            "bound": "hbm",
-            "kernel": dominant,
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profiles(dominant, args.workload),
+        traffic = traffic_from_profiles(dominant, args.workload)
+        hbm = None
+        if traffic is not None:
+            hbm_gbs = traffic / avg_s / 1e9
+            hbm = {"achieved": round(hbm_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                   "frac": round(hbm_gbs / HBM_PEAK_GBS, 4)}
+        valu = valu_from_profiles(dominant, args.workload)
+        ny, nx = wl.shape
+        lane_steps = ny * nx * (wl.kernel_len - 1)  # pixel-steps of one launch (one pass)
+        if valu is not None:
+            lane_instr = valu["insts"] * 64
+            achieved = lane_instr / avg_s / 1e12
+            roofline = {
+                "bound": "valu",
+                "kernel": dominant,
+                "achieved": round(achieved, 3),
+                "peak": VALU_PEAK_TLANE,
+                "unit": "T VALU lane-instructions/s",
+                "frac": round(achieved / VALU_PEAK_TLANE, 4),
+                "traffic": traffic,
+                "valu_lane_instructions_per_launch": lane_instr,
+                "valu_per_lane_step": round(lane_instr / lane_steps, 2),
+                "hbm": hbm,
+            }
+        else:  # no counter summary for this kernel: the HBM view only
+            hbm_gbs = (traffic or 0.0) / avg_s / 1e9
+            roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(hbm_gbs, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(hbm_gbs / HBM_PEAK_GBS, 4), "traffic": traffic}
+        roofline.update({
+            "logical_gather_gbs": round(b_alg / avg_s / 1e9, 1),
             "algorithmic_bytes_per_launch": b_alg,
             "avg_launch_ms": round(avg_s * 1e3, 4),
             "kernel_ms_per_step": {k: round(v["total_ms"] / args.steps, 4)
                                    for k, v in prof.items()},
-        }
+        })
 
     if roofline is not None:
         # untimed: what a plain copy reaches on this HBM, beside the nominal peak
@@ -408,6 +566,12 @@ def main() -> int:
         log(f"bench: CPU oracle baseline on {args.cpu_rows} rows")
         cpu = cpu_baseline(wl, args.cpu_rows)
 
+    extras = None
+    if not args.no_extras and args.workload == "plain4096":
+        del t
+        torch.cuda.empty_cache()
+        extras = extras_measure(dev, args)
+
     ny, nx = wl.shape
     line = {
         "metric": "Gpixel-steps/s",
@@ -425,7 +589,7 @@ def main() -> int:
         "config": {
             "workload": wl.name,
             "shape": [ny, nx],
-            "kernel_len": int(wl.kernel.size),
+            "kernel_len": wl.kernel_len,
             "iterations": wl.iterations,
             "boundaries": wl.boundaries,
             "uv_mode": wl.uv_mode,
@@ -434,6 +598,7 @@ def main() -> int:
         "roofline": roofline,
         "divergence": divergence,
         "cpu_baseline": cpu,
+        "extras": extras,
     }
     print(json.dumps(line), flush=True)
     return 0

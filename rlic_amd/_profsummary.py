@@ -5,6 +5,8 @@
                                             <WRITE_SIZE counter_collection.csv>
                                             [--workload NAME] [--out-dir DIR]
     python -m rlic_amd._profsummary counters <counter_collection.csv> ... [--templates]
+    python -m rlic_amd._profsummary valu <counter_collection.csv> ...
+                                         [--workload NAME] [--out-dir DIR]
 
 `stats` prints the per-kernel launch count and average duration.
 `traffic` turns the per-dispatch FETCH_SIZE / WRITE_SIZE counters (KiB)
@@ -20,6 +22,10 @@ SQ_LDS_IDX_ACTIVE, L2 hit rate TCC_HIT / (TCC_HIT + TCC_MISS), L1 hit rate
 1 - TCP_TCC_READ_REQ / TCP_TOTAL_ACCESSES, VALU / SALU / LDS / vector-read
 instructions per wave, and the share of wave cycles spent waiting
 SQ_WAIT_INST_ANY / SQ_WAVE_CYCLES.
+`valu` reads SQ_INSTS_VALU and SQ_WAVES (and, when present, GRBM_GUI_ACTIVE,
+SQ_BUSY_CYCLES, SQ_ACTIVE_INST_VALU) per dispatch and, with --out-dir, writes
+valu_<kernel>_<workload>.json with the wave64 VALU instructions per launch,
+which bench.py turns into the VALU issue rate of its roofline.
 With --templates, kernels are told apart by their template arguments
 (e.g. the tile shapes of lic_replay_lds) instead of by name alone.
 """
@@ -134,6 +140,32 @@ def counter_ratios(paths: list[Path], templates: bool = False) -> dict[str, dict
     return out
 
 
+def valu(paths: list[Path]) -> dict[str, dict]:
+    """Per kernel: wave64 VALU instructions and waves per launch (averages
+    over its dispatches), with the other counters present, per launch."""
+    per: dict[str, dict[str, dict[str, float]]] = defaultdict(lambda: defaultdict(dict))
+    for path in paths:
+        with open(path, newline="") as f:
+            for row in csv.DictReader(f):
+                name = short_name(row["Kernel_Name"])
+                counter = row["Counter_Name"].removesuffix("_sum")
+                key = f"{path}:{row['Dispatch_Id']}"
+                d = per[name][key]
+                d[counter] = d.get(counter, 0.0) + float(row["Counter_Value"])
+    out = {}
+    for name, disp in per.items():
+        rows = [d for d in disp.values() if "SQ_INSTS_VALU" in d and "SQ_WAVES" in d]
+        if not rows:
+            continue
+        avg = {c: sum(d.get(c, 0.0) for d in rows) / len(rows)
+               for c in sorted({c for d in rows for c in d})}
+        out[name] = {"valu_insts_per_launch": avg["SQ_INSTS_VALU"],
+                     "waves_per_launch": avg["SQ_WAVES"],
+                     "valu_insts_per_wave": avg["SQ_INSTS_VALU"] / avg["SQ_WAVES"],
+                     "counters_per_launch": avg, "dispatches": len(rows)}
+    return out
+
+
 def main(argv: list[str] | None = None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -148,7 +180,22 @@ def main(argv: list[str] | None = None) -> int:
     c = sub.add_parser("counters")
     c.add_argument("csv", type=Path, nargs="+")
     c.add_argument("--templates", action="store_true")
+    v = sub.add_parser("valu")
+    v.add_argument("csv", type=Path, nargs="+")
+    v.add_argument("--workload", default="plain4096")
+    v.add_argument("--out-dir", type=Path)
     args = ap.parse_args(argv)
+    if args.cmd == "valu":
+        for name, r in sorted(valu(args.csv).items()):
+            if not name.startswith("lic_"):
+                continue
+            print(f"{name:24s} dispatches {r['dispatches']}, VALU/launch "
+                  f"{r['valu_insts_per_launch']:.4g}, waves {r['waves_per_launch']:.0f}, "
+                  f"VALU/wave {r['valu_insts_per_wave']:.1f}")
+            if args.out_dir:
+                path = args.out_dir / f"valu_{name}_{args.workload}.json"
+                path.write_text(json.dumps(r, indent=1) + "\n")
+        return 0
     if args.cmd == "counters":
         for name, r in sorted(counter_ratios(args.csv, args.templates).items()):
             if not name.startswith("lic_"):

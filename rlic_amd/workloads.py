@@ -39,17 +39,26 @@ class Workload:
 
     @property
     def shape(self) -> tuple[int, int]:
-        return tuple(self.texture.shape)
+        return tuple(int(n) for n in self.texture.shape)
+
+    @property
+    def kernel_len(self) -> int:
+        return int(self.kernel.shape[0])
+
+    @property
+    def itemsize(self) -> int:
+        t = self.texture
+        return int(t.element_size()) if hasattr(t, "element_size") else int(t.itemsize)
 
     @property
     def pixel_steps(self) -> int:
         ny, nx = self.shape
-        return ny * nx * (self.kernel.size - 1) * self.iterations
+        return ny * nx * (self.kernel_len - 1) * self.iterations
 
     def algorithmic_bytes_per_pass(self) -> int:
         """Reference gathers per pass: u, v, texture per step + centre read + write."""
         ny, nx = self.shape
-        return ny * nx * (3 * (self.kernel.size - 1) + 2) * self.texture.itemsize
+        return ny * nx * (3 * (self.kernel_len - 1) + 2) * self.itemsize
 
     def kwargs(self) -> dict:
         return dict(kernel=self.kernel, uv_mode=self.uv_mode, boundaries=self.boundaries,
@@ -261,3 +270,28 @@ WORKLOADS = {
     "pol16384": lambda: polarization_field(16384, np.float32),
     "pol8192f64": lambda: polarization_field(8192, np.float64),
 }
+
+
+def polarization_field_device(n: int = 16384, dtype=np.float32, seed: int = 3,
+                              device="cuda") -> Workload:
+    """``polarization_field`` generated on a ROCm device (no host arrays):
+    texture from the counter-based noise of ``rlic_amd.generate.noise``
+    (seeded), U = -1 left half / +1 right half plus 0.3 x modes, V = modes,
+    the modes drawn from ``default_rng(seed)`` as ``polarization_field``
+    draws them; closed, polarization, triangular kernel of 257 taps, one
+    pass.  The arrays are torch tensors on ``device``."""
+    import torch
+
+    from rlic_amd import generate
+
+    rng = np.random.default_rng(seed)
+    dev = torch.device(device)
+    texture = generate.noise((n, n), seed=seed, dtype=dtype, device=dev)
+    u = generate.random_fourier_modes(rng, (n, n), dtype=dtype, device=dev)
+    v = generate.random_fourier_modes(rng, (n, n), dtype=dtype, device=dev)
+    tdt = texture.dtype
+    half = torch.where(torch.arange(n, device=dev) < n / 2, -1.0, 1.0).to(tdt)
+    u.mul_(0.3).add_(half[None, :])
+    kernel = torch.from_numpy((1 - np.abs(np.linspace(-1, 1, 257))).astype(dtype)).to(dev)
+    return Workload(f"polarization_{n}x{n}_{np.dtype(dtype).name}_L257", texture, u, v, kernel,
+                    "closed", "polarization", 1, meta={"generated_on": "device"})
