@@ -136,6 +136,25 @@ int rlic_convolve_noalloc_f64(const double* texture, int64_t tex_sy, int64_t tex
                               double* out,
                               int profile, void* stream);
 
+/* Global histogram equalization of an image: the step usually applied to a
+ * LIC image, run where that image lies (with RLIC_DEVICE, on the output of
+ * a device rlic_convolve_* call, which never leaves HBM).  Replaces the
+ * reference's declared but unimplemented rlic._core.equalize_histogram_f32
+ * / _f64(image, nbins) (src/rlic/_core.pyi:30-37; README.md:19-23 points
+ * to the sister project `ahe`).  out(x) = cdf(bin(x)) / N over the finite
+ * pixels: vmin, vmax the finite range, bin(x) = min(floor((x - vmin) *
+ * (nbins / (vmax - vmin))), nbins - 1) in double (bin 0 when vmax == vmin),
+ * N the finite pixel count; NaN and +-inf pixels are copied unchanged.
+ * `image` has element strides (sy, sx); `out` is C-order ny x nx and may be
+ * `image` itself when that is C-contiguous.  nbins in [1, 2^24].  Same
+ * `location` / `stream` conventions as rlic_convolve_*. */
+int rlic_equalize_histogram_f32(const float* image, int64_t sy, int64_t sx, int64_t ny,
+                                int64_t nx, int64_t nbins, float* out, int location,
+                                void* stream);
+int rlic_equalize_histogram_f64(const double* image, int64_t sy, int64_t sx, int64_t ny,
+                                int64_t nx, int64_t nbins, double* out, int location,
+                                void* stream);
+
 /* One field of a batched call: the arguments of one rlic_convolve_* call
  * with DEVICE pointers (see rlic_convolve_batch_*). */
 typedef struct {

@@ -2,6 +2,9 @@
 
 Drop-in for ``rlic.convolve`` (reference ``src/rlic/__init__.py:1-7``):
 ``import rlic_amd as rlic`` and call ``rlic.convolve(...)`` unchanged.
+The public surface is the reference's (``__all__``); beyond it,
+``rlic_amd.equalize_histogram`` post-processes a LIC image where it lies
+(device tensors stay on the device).
 """
 
 __all__ = [
@@ -9,3 +12,4 @@ __all__ = [
 ]
 
 from rlic_amd._lib import convolve
+from rlic_amd.equalize import equalize_histogram  # noqa: F401

@@ -21,9 +21,11 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 SOURCES = [CSRC / "lic_kernels.hip", CSRC / "lic_scratch.hip", CSRC / "lic_batch.hip",
-           CSRC / "lic_serve.hip", CSRC / "lic_generate.hip", CSRC / "lic_api.hip"]
+           CSRC / "lic_serve.hip", CSRC / "lic_generate.hip", CSRC / "lic_equalize.hip",
+           CSRC / "lic_api.hip"]
 HEADERS = [CSRC / "lic_device.h", CSRC / "lic_launch.h", CSRC / "lic_steps.h", CSRC / "lic_batch.h",
-           CSRC / "lic_serve.h", CSRC / "lic_scratch.h", ROOT / "include" / "rlic_hip.h"]
+           CSRC / "lic_serve.h", CSRC / "lic_scratch.h", CSRC / "lic_equalize.h",
+           ROOT / "include" / "rlic_hip.h"]
 OUTPUT = PKG / "librlic_hip.so"
 ARCH = os.environ.get("RLIC_OFFLOAD_ARCH", "gfx950")
 

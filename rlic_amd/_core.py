@@ -22,7 +22,8 @@ ValueError (the reference aborts the process there, src/lib.rs:382).
 
 from __future__ import annotations
 
-__all__ = ["ARITHMETIC_PROFILES", "convolve_f32", "convolve_f64"]
+__all__ = ["ARITHMETIC_PROFILES", "convolve_f32", "convolve_f64", "equalize_histogram_f32",
+           "equalize_histogram_f64"]
 
 import ctypes
 from typing import Any
@@ -212,3 +213,60 @@ def convolve_f64(texture, uv, kernel, boundaries, iterations=1, *, profile=None)
     return _convolve(
         np.dtype("float64"), "rlic_convolve_f64", texture, uv, kernel, boundaries, iterations, profile
     )
+
+
+def _check_nbins(nbins: Any) -> int:
+    try:
+        n = int(nbins.__index__())
+    except AttributeError:
+        raise TypeError(
+            f"argument 'nbins': {type(nbins).__name__!r} object cannot be interpreted as an "
+            "integer"
+        ) from None
+    if n < 1:
+        raise ValueError(f"nbins must be a strictly positive integer, got {n}")
+    if n > 1 << 24:
+        raise ValueError(f"nbins must be at most {1 << 24}, got {n}")
+    return n
+
+
+def _equalize(dtype: np.dtype, entry: str, image: Any, nbins: Any) -> Any:
+    nb = _check_nbins(nbins)
+    fn = getattr(_native.load(), entry)
+    if _is_torch(image):
+        import torch
+
+        tdtype = {np.dtype("float32"): torch.float32, np.dtype("float64"): torch.float64}[dtype]
+        if image.dtype != tdtype:
+            raise TypeError(f"argument 'image': type mismatch: expected {tdtype}, got {image.dtype}")
+        if image.dim() != 2:
+            raise TypeError(f"argument 'image': dimensionality mismatch: expected 2, got {image.dim()}")
+        if image.device.type == "cpu":
+            return torch.from_numpy(_equalize(dtype, entry, image.numpy(), nb))
+        ny, nx = image.shape
+        with torch.cuda.device(image.device):
+            out = torch.empty((ny, nx), dtype=tdtype, device=image.device)
+            stream = torch.cuda.current_stream(image.device).cuda_stream
+            status = fn(image.data_ptr(), image.stride(0), image.stride(1), ny, nx, nb,
+                        out.data_ptr(), _native.RLIC_DEVICE, ctypes.c_void_p(stream))
+        _native.raise_for_status(status)
+        return out
+    img, st = _numpy_view(image, "image", dtype, 2)
+    ny, nx = img.shape
+    out = np.empty((ny, nx), dtype=dtype)
+    status = fn(img.ctypes.data, st[0], st[1], ny, nx, nb, out.ctypes.data, _native.RLIC_HOST,
+                None)
+    _native.raise_for_status(status)
+    return out
+
+
+def equalize_histogram_f32(image, nbins):
+    """float32 global histogram equalization; the reference declares this
+    function (``src/rlic/_core.pyi:30-33``) but implements it nowhere."""
+    return _equalize(np.dtype("float32"), "rlic_equalize_histogram_f32", image, nbins)
+
+
+def equalize_histogram_f64(image, nbins):
+    """float64 global histogram equalization (declared, unimplemented, in
+    the reference: ``src/rlic/_core.pyi:34-37``)."""
+    return _equalize(np.dtype("float64"), "rlic_equalize_histogram_f64", image, nbins)

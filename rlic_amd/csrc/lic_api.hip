@@ -26,6 +26,7 @@
 
 #include "../../include/rlic_hip.h"
 #include "lic_batch.h"
+#include "lic_equalize.h"
 #include "lic_launch.h"
 #include "lic_scratch.h"
 #include "lic_serve.h"
@@ -928,6 +929,65 @@ int convolve_noalloc_entry(const T* tex, int64_t tsy, int64_t tsx, const T* u, i
                           &fixed);
 }
 
+// Largest bin count of an equalization (the counters are 4 bytes each).
+constexpr int64_t kMaxBins = int64_t{1} << 24;
+
+template <typename T>
+int equalize_entry(const T* image, int64_t sy, int64_t sx, int64_t ny, int64_t nx, int64_t nbins,
+                   T* out, int location, void* stream_ptr) {
+  t_last_error.clear();
+  if (ny < 0 || nx < 0) return set_error(RLIC_ERR_INVALID_ARGUMENT, "negative image size");
+  if (nbins < 1 || nbins > kMaxBins)
+    return set_error(RLIC_ERR_INVALID_ARGUMENT, "nbins must be in [1, %lld], got %lld",
+                     static_cast<long long>(kMaxBins), static_cast<long long>(nbins));
+  if (location != RLIC_HOST && location != RLIC_DEVICE)
+    return set_error(RLIC_ERR_INVALID_ARGUMENT, "unknown memory location %d", location);
+  if (ny == 0 || nx == 0) return RLIC_OK;  // an empty image: nothing to do
+  if (ny > INT32_MAX || nx > INT32_MAX || ny * nx > INT32_MAX)
+    return set_error(RLIC_ERR_INVALID_ARGUMENT,
+                     "image of %lld x %lld pixels exceeds the supported 2^31-1 pixels",
+                     static_cast<long long>(ny), static_cast<long long>(nx));
+  if (!image || !out) return set_error(RLIC_ERR_INVALID_ARGUMENT, "null array pointer");
+  int dev = 0;
+  RLIC_HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+  configure_pool(dev);
+  hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+  const int64_t n = ny * nx;
+  const int32_t bins = static_cast<int32_t>(nbins);
+  auto run = [&](const T* img, int64_t isy, int64_t isx, T* dst) -> int {
+    rlic::CallScratch scratch;
+    RLIC_HIP_TRY(scratch.get(rlic::equalize_scratch_bytes<T>(bins), stream),
+                 "allocating equalization scratch");
+    LaunchTimer timer("lic_equalize", stream);
+    RLIC_HIP_TRY(rlic::launch_equalize<T>(img, isy, isx, static_cast<int32_t>(ny),
+                                          static_cast<int32_t>(nx), bins, dst, scratch.ptr(),
+                                          stream),
+                 "equalize");
+    return RLIC_OK;
+  };
+  if (location == RLIC_DEVICE) return run(image, sy, sx, out);
+  // host arrays: upload the memory extent of the strided view, run, download
+  DevMem d_in, d_out;
+  int64_t lo, hi;
+  view_span(ny, nx, sy, sx, lo, hi);
+  const size_t count = static_cast<size_t>(hi - lo + 1);
+  RLIC_HIP_TRY(d_in.alloc(sizeof(T) * count, stream), "allocating input");
+  RLIC_HIP_TRY(hipMemcpyAsync(d_in.ptr, image + lo, sizeof(T) * count, hipMemcpyHostToDevice,
+                              stream),
+               "uploading input");
+  RLIC_HIP_TRY(d_out.alloc(sizeof(T) * static_cast<size_t>(n), stream), "allocating output");
+  const int rc = run(static_cast<const T*>(d_in.ptr) - lo, sy, sx, static_cast<T*>(d_out.ptr));
+  if (rc != RLIC_OK) {
+    (void)hipStreamSynchronize(stream);
+    return rc;
+  }
+  RLIC_HIP_TRY(hipMemcpyAsync(out, d_out.ptr, sizeof(T) * static_cast<size_t>(n),
+                              hipMemcpyDeviceToHost, stream),
+               "downloading output");
+  RLIC_HIP_TRY(hipStreamSynchronize(stream), "synchronizing");
+  return RLIC_OK;
+}
+
 template <typename T>
 int server_read(rlic::Server* server, int64_t handle, T* out) {
   t_last_error.clear();
@@ -1073,6 +1133,18 @@ int rlic_convolve_noalloc_f64(const double* texture, int64_t tex_sy, int64_t tex
   return convolve_noalloc_entry<double>(texture, tex_sy, tex_sx, u, u_sy, u_sx, v, v_sy, v_sx,
                                         ny, nx, kernel, kernel_len, uv_mode, x_left, x_right,
                                         y_left, y_right, iterations, out, profile, stream);
+}
+
+int rlic_equalize_histogram_f32(const float* image, int64_t sy, int64_t sx, int64_t ny,
+                                int64_t nx, int64_t nbins, float* out, int location,
+                                void* stream) {
+  return equalize_entry<float>(image, sy, sx, ny, nx, nbins, out, location, stream);
+}
+
+int rlic_equalize_histogram_f64(const double* image, int64_t sy, int64_t sx, int64_t ny,
+                                int64_t nx, int64_t nbins, double* out, int location,
+                                void* stream) {
+  return equalize_entry<double>(image, sy, sx, ny, nx, nbins, out, location, stream);
 }
 
 int rlic_convolve_batch_f32(const rlic_field_f32* fields, int64_t n_fields, int profile,
