@@ -323,3 +323,21 @@ def test_dlpack_inputs_are_validated():
     k = _DLPackOnly(np.ones(3, dtype="int64"))
     with pytest.raises(TypeError, match=r"Found unsupported data type\(s\)"):
         rlic.convolve(img, img, img, kernel=k)
+
+
+def test_reference_import_name():
+    """``import rlic`` (the reference's name) gives this build's convolve and
+    the submodules the reference's tests import."""
+    import rlic
+    import rlic_amd
+    from rlic._boundaries import BoundarySet, SUPPORTED_BOUNDS
+    from rlic._core import convolve_f32, convolve_f64
+    from rlic._typing import D2, F, FArray1D, FArray2D, f64  # noqa: F401
+
+    assert rlic.__all__ == ["convolve"] and rlic.convolve is rlic_amd.convolve
+    assert BoundarySet is rlic_amd._boundaries.BoundarySet
+    assert SUPPORTED_BOUNDS == {"closed", "periodic"}
+    assert convolve_f32 is rlic_amd._core.convolve_f32 and callable(convolve_f64)
+    with pytest.raises(ValueError, match="^Invalid number of iterations"):
+        rlic.convolve(np.ones((4, 4)), np.ones((4, 4)), np.ones((4, 4)), kernel=np.ones(3),
+                      iterations=-1)
