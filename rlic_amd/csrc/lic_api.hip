@@ -14,6 +14,7 @@
 // traced again instead (texture slot re-packed between passes).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
@@ -693,6 +694,74 @@ int validate(const T* tex, const T* u, const T* v, int64_t ny, int64_t nx, const
   return RLIC_OK;
 }
 
+// Side streams of batched calls: per device, default priority,
+// non-blocking, created on first use and kept until the process ends.
+// Work enqueued on them is ordered after the caller's stream by an event
+// (start) and joined back into it by one event per side stream (join).
+struct SideWork {
+  hipStream_t caller;
+  std::vector<hipStream_t> streams;
+  SideWork(hipStream_t s) : caller(s) {}
+  SideWork(const SideWork&) = delete;
+  SideWork& operator=(const SideWork&) = delete;
+
+  bool start(int dev, int k) {
+    static std::mutex mu;
+    static std::map<int, std::vector<hipStream_t>> pool;
+    {
+      std::lock_guard<std::mutex> lock(mu);
+      std::vector<hipStream_t>& p = pool[dev];
+      while (static_cast<int>(p.size()) < k) {
+        hipStream_t s = nullptr;
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+          (void)hipGetLastError();
+          return false;
+        }
+        p.push_back(s);
+      }
+      streams.assign(p.begin(), p.begin() + k);
+    }
+    hipEvent_t ready = nullptr;
+    bool ok = hipEventCreateWithFlags(&ready, hipEventDisableTiming) == hipSuccess &&
+              hipEventRecord(ready, caller) == hipSuccess;
+    for (hipStream_t s : streams) ok = ok && hipStreamWaitEvent(s, ready, 0) == hipSuccess;
+    if (ready) (void)hipEventDestroy(ready);  // released once the waits are satisfied
+    if (!ok) {
+      (void)hipGetLastError();
+      finish();
+      return false;
+    }
+    return true;
+  }
+
+  // Makes the caller's stream wait for everything enqueued on the side
+  // streams so far; the first error, if any.
+  hipError_t join() {
+    hipError_t first = hipSuccess;
+    for (hipStream_t s : streams) {
+      hipEvent_t done = nullptr;
+      hipError_t e = hipEventCreateWithFlags(&done, hipEventDisableTiming);
+      if (e == hipSuccess) e = hipEventRecord(done, s);
+      if (e == hipSuccess) e = hipStreamWaitEvent(caller, done, 0);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        e = hipStreamSynchronize(s);  // no event: wait on the host instead
+      }
+      if (done) (void)hipEventDestroy(done);
+      if (e != hipSuccess && first == hipSuccess) first = e;
+    }
+    streams.clear();
+    return first;
+  }
+
+  // Error path: waits on the host for the side streams' work.
+  void finish() {
+    for (hipStream_t s : streams) (void)hipStreamSynchronize(s);
+    (void)hipGetLastError();
+    streams.clear();
+  }
+};
+
 template <typename T, typename FieldSpec>
 int batch_entry(const FieldSpec* fields, int64_t nfields, int profile, void* stream_ptr) {
   t_last_error.clear();
@@ -720,45 +789,88 @@ int batch_entry(const FieldSpec* fields, int64_t nfields, int profile, void* str
   configure_pool(dev);
   hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
   // Fields of at least RLIC_BATCH_SINGLE_MIN pixels (default 2^18; 0 or
-  // less: none) fill the GPU on their own: they run as single-field launches
-  // (trace, then LDS-staged or tuned replays), ahead of one persistent
-  // launch shared by the rest.  Either way a field gives the bits of its
-  // single call.
+  // less: none) run as single-field launches (trace, then LDS-staged or
+  // tuned replays); the rest share one persistent launch.  Either way a
+  // field gives the bits of its single call.
   const int single_min = env_int("RLIC_BATCH_SINGLE_MIN", 1 << 18);
   std::vector<FieldSpec> rest;
   rest.reserve(static_cast<size_t>(nfields));
+  std::vector<int64_t> big;
+  for (int64_t k = 0; k < nfields; ++k) {
+    const FieldSpec& f = fields[k];
+    // an empty image has an empty output: no work, and its other side may
+    // be any size (no per-field state is sized from it)
+    if (f.ny == 0 || f.nx == 0) continue;
+    if (single_min <= 0 || f.ny * f.nx < single_min) rest.push_back(f);
+    else big.push_back(k);
+  }
+  // Single-field launches of a field under about 2^20 pixels fill only part
+  // of the GPU, so the big fields are spread over up to RLIC_BATCH_STREAMS
+  // (default 4; 1: the caller's stream only) side streams, largest first to
+  // the least loaded one, where they overlap with each other and with the
+  // persistent launch on the caller's stream.
+  const int64_t want_side =
+      std::min<int64_t>(std::max(1, std::min(env_int("RLIC_BATCH_STREAMS", 4), 16)),
+                        static_cast<int64_t>(big.size()));
+  SideWork side(stream);
+  if (want_side > 1) {
+    if (!side.start(dev, static_cast<int>(want_side))) side.streams.clear();
+  }
   rlic::BatchScratch scratch;
   std::string err;
   int rc = RLIC_OK;
   {
     LaunchTimer timer("lic_batch", stream);
-    for (int64_t k = 0; k < nfields && rc == RLIC_OK; ++k) {
-      const FieldSpec& f = fields[k];
-      // an empty image has an empty output: no work, and its other side may
-      // be any size (no per-field state is sized from it)
-      if (f.ny == 0 || f.nx == 0) continue;
-      if (single_min <= 0 || f.ny * f.nx < single_min) {
-        rest.push_back(f);
-        continue;
+    std::vector<int> lane_of(big.size(), 0);
+    if (!side.streams.empty()) {
+      std::vector<size_t> order(big.size());
+      for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+      auto cost = [&](size_t i) {
+        const FieldSpec& f = fields[big[i]];
+        return f.ny * f.nx * std::max<int64_t>(f.iterations, 1);
+      };
+      std::sort(order.begin(), order.end(),
+                [&](size_t a, size_t b) { return cost(a) > cost(b); });
+      std::vector<int64_t> load(side.streams.size(), 0);
+      for (size_t i : order) {
+        const size_t s = static_cast<size_t>(
+            std::min_element(load.begin(), load.end()) - load.begin());
+        lane_of[i] = static_cast<int>(s);
+        load[s] += cost(i);
       }
+    }
+    for (size_t i = 0; i < big.size() && rc == RLIC_OK; ++i) {
+      const int64_t k = big[i];
+      const FieldSpec& f = fields[k];
+      const hipStream_t s = side.streams.empty() ? stream : side.streams[lane_of[i]];
       const rlic::FieldGeom g{static_cast<int32_t>(f.ny), static_cast<int32_t>(f.nx),
                               f.x_left, f.x_right, f.y_left, f.y_right};
       rc = run_on_device<T>(f.texture, f.tex_sy, f.tex_sx, f.u, f.u_sy, f.u_sx, f.v, f.v_sy,
                             f.v_sx, static_cast<int32_t>(f.ny), static_cast<int32_t>(f.nx),
                             f.kernel, static_cast<int32_t>(f.kernel_len), f.uv_mode, g,
-                            f.iterations, f.out, profile, stream);
+                            f.iterations, f.out, profile, s);
       if (rc != RLIC_OK) {
         t_last_error = "field " + std::to_string(k) + ": " + t_last_error;
+        side.finish();
+        (void)hipStreamSynchronize(stream);
         return rc;
       }
     }
     if (!rest.empty()) {
       rc = rlic::batch_run<T, FieldSpec>(rest.data(), static_cast<int64_t>(rest.size()), profile,
                                          stream, scratch, err);
-    } else if (hipError_t e = hipStreamSynchronize(stream); e != hipSuccess) {
-      (void)hipGetLastError();
+    }
+    // the caller's stream (and the batch timer) end after the side work
+    if (hipError_t e = side.join(); e != hipSuccess && rc == RLIC_OK) {
       rc = RLIC_ERR_HIP;
       err = std::string("batched call: ") + hipGetErrorString(e);
+    }
+    if (rc == RLIC_OK) {
+      if (hipError_t e = hipStreamSynchronize(stream); e != hipSuccess) {
+        (void)hipGetLastError();
+        rc = RLIC_ERR_HIP;
+        err = std::string("batched call: ") + hipGetErrorString(e);
+      }
     }
   }
   if (rc != RLIC_OK) {

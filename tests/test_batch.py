@@ -50,12 +50,15 @@ def test_batch_matches_single_calls():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("streams", ["1", "3", "8"])
 @pytest.mark.parametrize("single_min", ["1", "3000"])
-def test_batch_single_field_launches_match_single_calls(monkeypatch, single_min):
+def test_batch_single_field_launches_match_single_calls(monkeypatch, single_min, streams):
     """Fields at or above RLIC_BATCH_SINGLE_MIN pixels run as single-field
-    launches ahead of the persistent launch of the rest: every field, and
-    the mix, gives the single calls' bits."""
+    launches, spread over RLIC_BATCH_STREAMS side streams, next to the
+    persistent launch of the rest: every field, and the mix, gives the
+    single calls' bits."""
     monkeypatch.setenv("RLIC_BATCH_SINGLE_MIN", single_min)
+    monkeypatch.setenv("RLIC_BATCH_STREAMS", streams)
     rng = np.random.default_rng(5)
     fields = _mixed_fields(rng, 24)
     sizes = [f.texture.size for f in fields]
