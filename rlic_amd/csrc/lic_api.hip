@@ -805,10 +805,11 @@ int batch_entry(const FieldSpec* fields, int64_t nfields, int profile, void* str
     else big.push_back(k);
   }
   // Single-field launches of a field under about 2^20 pixels fill only part
-  // of the GPU, so the big fields are spread over up to RLIC_BATCH_STREAMS
-  // (default 4; 1: the caller's stream only) side streams, largest first to
-  // the least loaded one, where they overlap with each other and with the
-  // persistent launch on the caller's stream.
+  // of the GPU and leave it idle in their tails, so the big fields are
+  // spread over up to RLIC_BATCH_STREAMS (default 4; 1: the caller's stream
+  // only) side streams, largest first to the least loaded one, where they
+  // overlap with each other and with the persistent launch on the caller's
+  // stream.
   const int64_t want_side =
       std::min<int64_t>(std::max(1, std::min(env_int("RLIC_BATCH_STREAMS", 4), 16)),
                         static_cast<int64_t>(big.size()));
