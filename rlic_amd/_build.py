@@ -25,6 +25,7 @@ SOURCES = [CSRC / "lic_kernels.hip", CSRC / "lic_scratch.hip", CSRC / "lic_batch
            CSRC / "lic_api.hip"]
 HEADERS = [CSRC / "lic_device.h", CSRC / "lic_launch.h", CSRC / "lic_steps.h", CSRC / "lic_batch.h",
            CSRC / "lic_serve.h", CSRC / "lic_scratch.h", CSRC / "lic_equalize.h",
+           CSRC / "lic_streams.h",
            ROOT / "include" / "rlic_hip.h"]
 OUTPUT = PKG / "librlic_hip.so"
 ARCH = os.environ.get("RLIC_OFFLOAD_ARCH", "gfx950")

@@ -31,6 +31,7 @@
 #include "lic_launch.h"
 #include "lic_scratch.h"
 #include "lic_serve.h"
+#include "lic_streams.h"
 
 namespace {
 
@@ -694,74 +695,6 @@ int validate(const T* tex, const T* u, const T* v, int64_t ny, int64_t nx, const
   return RLIC_OK;
 }
 
-// Side streams of batched calls: per device, default priority,
-// non-blocking, created on first use and kept until the process ends.
-// Work enqueued on them is ordered after the caller's stream by an event
-// (start) and joined back into it by one event per side stream (join).
-struct SideWork {
-  hipStream_t caller;
-  std::vector<hipStream_t> streams;
-  SideWork(hipStream_t s) : caller(s) {}
-  SideWork(const SideWork&) = delete;
-  SideWork& operator=(const SideWork&) = delete;
-
-  bool start(int dev, int k) {
-    static std::mutex mu;
-    static std::map<int, std::vector<hipStream_t>> pool;
-    {
-      std::lock_guard<std::mutex> lock(mu);
-      std::vector<hipStream_t>& p = pool[dev];
-      while (static_cast<int>(p.size()) < k) {
-        hipStream_t s = nullptr;
-        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
-          (void)hipGetLastError();
-          return false;
-        }
-        p.push_back(s);
-      }
-      streams.assign(p.begin(), p.begin() + k);
-    }
-    hipEvent_t ready = nullptr;
-    bool ok = hipEventCreateWithFlags(&ready, hipEventDisableTiming) == hipSuccess &&
-              hipEventRecord(ready, caller) == hipSuccess;
-    for (hipStream_t s : streams) ok = ok && hipStreamWaitEvent(s, ready, 0) == hipSuccess;
-    if (ready) (void)hipEventDestroy(ready);  // released once the waits are satisfied
-    if (!ok) {
-      (void)hipGetLastError();
-      finish();
-      return false;
-    }
-    return true;
-  }
-
-  // Makes the caller's stream wait for everything enqueued on the side
-  // streams so far; the first error, if any.
-  hipError_t join() {
-    hipError_t first = hipSuccess;
-    for (hipStream_t s : streams) {
-      hipEvent_t done = nullptr;
-      hipError_t e = hipEventCreateWithFlags(&done, hipEventDisableTiming);
-      if (e == hipSuccess) e = hipEventRecord(done, s);
-      if (e == hipSuccess) e = hipStreamWaitEvent(caller, done, 0);
-      if (e != hipSuccess) {
-        (void)hipGetLastError();
-        e = hipStreamSynchronize(s);  // no event: wait on the host instead
-      }
-      if (done) (void)hipEventDestroy(done);
-      if (e != hipSuccess && first == hipSuccess) first = e;
-    }
-    streams.clear();
-    return first;
-  }
-
-  // Error path: waits on the host for the side streams' work.
-  void finish() {
-    for (hipStream_t s : streams) (void)hipStreamSynchronize(s);
-    (void)hipGetLastError();
-    streams.clear();
-  }
-};
-
 template <typename T, typename FieldSpec>
 int batch_entry(const FieldSpec* fields, int64_t nfields, int profile, void* stream_ptr) {
   t_last_error.clear();
@@ -813,10 +746,8 @@ int batch_entry(const FieldSpec* fields, int64_t nfields, int profile, void* str
   const int64_t want_side =
       std::min<int64_t>(std::max(1, std::min(env_int("RLIC_BATCH_STREAMS", 4), 16)),
                         static_cast<int64_t>(big.size()));
-  SideWork side(stream);
-  if (want_side > 1) {
-    if (!side.start(dev, static_cast<int>(want_side))) side.streams.clear();
-  }
+  rlic::SideWork side(stream);
+  if (want_side > 1) (void)side.start(dev, static_cast<int>(want_side));
   rlic::BatchScratch scratch;
   std::string err;
   int rc = RLIC_OK;
@@ -824,21 +755,12 @@ int batch_entry(const FieldSpec* fields, int64_t nfields, int profile, void* str
     LaunchTimer timer("lic_batch", stream);
     std::vector<int> lane_of(big.size(), 0);
     if (!side.streams.empty()) {
-      std::vector<size_t> order(big.size());
-      for (size_t i = 0; i < order.size(); ++i) order[i] = i;
-      auto cost = [&](size_t i) {
+      std::vector<int64_t> cost(big.size());
+      for (size_t i = 0; i < big.size(); ++i) {
         const FieldSpec& f = fields[big[i]];
-        return f.ny * f.nx * std::max<int64_t>(f.iterations, 1);
-      };
-      std::sort(order.begin(), order.end(),
-                [&](size_t a, size_t b) { return cost(a) > cost(b); });
-      std::vector<int64_t> load(side.streams.size(), 0);
-      for (size_t i : order) {
-        const size_t s = static_cast<size_t>(
-            std::min_element(load.begin(), load.end()) - load.begin());
-        lane_of[i] = static_cast<int>(s);
-        load[s] += cost(i);
+        cost[i] = f.ny * f.nx * std::max<int64_t>(f.iterations, 1);
       }
+      lane_of = rlic::SideWork::assign(cost, static_cast<int>(side.streams.size()));
     }
     for (size_t i = 0; i < big.size() && rc == RLIC_OK; ++i) {
       const int64_t k = big[i];

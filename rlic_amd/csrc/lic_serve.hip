@@ -16,12 +16,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include "../../include/rlic_hip.h"
 #include "lic_batch.h"
 #include "lic_launch.h"
 #include "lic_serve.h"
+#include "lic_streams.h"
 
 namespace rlic {
 
@@ -156,10 +158,30 @@ int Server::step_dtype(std::vector<int64_t>& finished, std::string& err) {
     stepped.push_back(&r);
     handles.push_back(kv.first);
   }
-  for (Resident<T>* r : singles) {
+  // The single-field replays go to up to RLIC_BATCH_STREAMS (default 4)
+  // side streams, largest first to the least loaded one, where they overlap
+  // with each other and with the persistent launch on the server's stream.
+  SideWork side(stream_);
+  std::vector<int> lane(singles.size(), 0);
+  if (singles.size() > 1) {
+    const char* env = std::getenv("RLIC_BATCH_STREAMS");
+    const int want = std::min<int>(std::min(env && *env ? std::atoi(env) : 4, 16),
+                                   static_cast<int>(singles.size()));
+    int dev = 0;
+    if (want > 1 && hipGetDevice(&dev) == hipSuccess && side.start(dev, want)) {
+      std::vector<int64_t> cost(singles.size());
+      for (size_t k = 0; k < singles.size(); ++k)
+        cost[k] = static_cast<int64_t>(singles[k]->g.ny) * singles[k]->g.nx * singles[k]->klen;
+      lane = SideWork::assign(cost, want);
+    }
+    (void)hipGetLastError();
+  }
+  for (size_t k = 0; k < singles.size(); ++k) {
+    Resident<T>* r = singles[k];
     const int out_buf = 1 - r->cur;
+    const hipStream_t s = side.streams.empty() ? stream_ : side.streams[lane[k]];
     if (hipError_t e = replay_pass<T>(r->buf[r->cur], r->buf[out_buf], r->weights, r->klen,
-                                      r->g, profile_, r->codes, stream_);
+                                      r->g, profile_, r->codes, s);
         e != hipSuccess) {
       (void)hipGetLastError();
       err = std::string("replay pass: ") + hipGetErrorString(e);
@@ -170,8 +192,11 @@ int Server::step_dtype(std::vector<int64_t>& finished, std::string& err) {
     const int rc = launch_queue<T>(fields, ksrc, segs, /*deps=*/false, /*pack=*/false, profile_,
                                    stream_, err);
     if (rc != RLIC_OK) return rc;
-  } else if (!singles.empty()) {
-    if (hipError_t e = hipStreamSynchronize(stream_); e != hipSuccess) {  // the step is synchronous
+  }
+  if (!singles.empty()) {  // the step is synchronous
+    hipError_t e = side.join();
+    if (hipError_t se = hipStreamSynchronize(stream_); e == hipSuccess) e = se;
+    if (e != hipSuccess) {
       (void)hipGetLastError();
       err = std::string("serving step: ") + hipGetErrorString(e);
       return RLIC_ERR_HIP;
