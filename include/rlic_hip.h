@@ -155,6 +155,38 @@ int rlic_equalize_histogram_f64(const double* image, int64_t sy, int64_t sx, int
                                 int64_t nx, int64_t nbins, double* out, int location,
                                 void* stream);
 
+/* Adaptive histogram equalization, replacing the reference's declared but
+ * unimplemented rlic._core.equalize_histogram_sliding_tile_f32/_f64 and
+ * equalize_histogram_tile_interpolation_f32/_f64(image, nbins, tile_shape)
+ * (src/rlic/_core.pyi:38-57).  Bins as in rlic_equalize_histogram_*
+ * (over the whole image's finite range); NaN and +-inf pixels copied.
+ * Sliding tile: out(p) = (finite pixels of p's window with a bin <= p's) /
+ * (finite pixels of the window), the window rows
+ * [i - (tile_h - 1) / 2, i - (tile_h - 1) / 2 + tile_h) and columns alike,
+ * clipped to the image (cost: tile_h * tile_w per pixel).  Tile
+ * interpolation: tiles of tile_h x tile_w from (0, 0), tile t mapping bin b
+ * to cdf_t(b) / N_t (the global mapping for a tile without finite pixels),
+ * bilinear interpolation between the mappings of the tiles whose centres
+ * surround the pixel (clamped at the edges), in double.  tile_h, tile_w
+ * >= 1; for tile interpolation tiles x nbins <= 2^30.  Other arguments as
+ * rlic_equalize_histogram_*. */
+int rlic_equalize_histogram_sliding_tile_f32(const float* image, int64_t sy, int64_t sx,
+                                             int64_t ny, int64_t nx, int64_t nbins,
+                                             int64_t tile_h, int64_t tile_w, float* out,
+                                             int location, void* stream);
+int rlic_equalize_histogram_sliding_tile_f64(const double* image, int64_t sy, int64_t sx,
+                                             int64_t ny, int64_t nx, int64_t nbins,
+                                             int64_t tile_h, int64_t tile_w, double* out,
+                                             int location, void* stream);
+int rlic_equalize_histogram_tile_interpolation_f32(const float* image, int64_t sy, int64_t sx,
+                                                   int64_t ny, int64_t nx, int64_t nbins,
+                                                   int64_t tile_h, int64_t tile_w, float* out,
+                                                   int location, void* stream);
+int rlic_equalize_histogram_tile_interpolation_f64(const double* image, int64_t sy, int64_t sx,
+                                                   int64_t ny, int64_t nx, int64_t nbins,
+                                                   int64_t tile_h, int64_t tile_w, double* out,
+                                                   int location, void* stream);
+
 /* One field of a batched call: the arguments of one rlic_convolve_* call
  * with DEVICE pointers (see rlic_convolve_batch_*). */
 typedef struct {

@@ -7,4 +7,8 @@ from rlic_amd._core import (  # noqa: F401
     convolve_f64,
     equalize_histogram_f32,
     equalize_histogram_f64,
+    equalize_histogram_sliding_tile_f32,
+    equalize_histogram_sliding_tile_f64,
+    equalize_histogram_tile_interpolation_f32,
+    equalize_histogram_tile_interpolation_f64,
 )

@@ -23,7 +23,9 @@ ValueError (the reference aborts the process there, src/lib.rs:382).
 from __future__ import annotations
 
 __all__ = ["ARITHMETIC_PROFILES", "convolve_f32", "convolve_f64", "equalize_histogram_f32",
-           "equalize_histogram_f64"]
+           "equalize_histogram_f64", "equalize_histogram_sliding_tile_f32",
+           "equalize_histogram_sliding_tile_f64", "equalize_histogram_tile_interpolation_f32",
+           "equalize_histogram_tile_interpolation_f64"]
 
 import ctypes
 from typing import Any
@@ -230,8 +232,22 @@ def _check_nbins(nbins: Any) -> int:
     return n
 
 
-def _equalize(dtype: np.dtype, entry: str, image: Any, nbins: Any) -> Any:
+def _check_tile_shape(tile_shape: Any) -> tuple[int, int]:
+    try:
+        th, tw = tile_shape
+        th, tw = int(th.__index__()), int(tw.__index__())
+    except (TypeError, ValueError, AttributeError):
+        raise TypeError(
+            f"argument 'tile_shape': expected a pair of integers, got {tile_shape!r}"
+        ) from None
+    if th < 1 or tw < 1:
+        raise ValueError(f"tile_shape must be strictly positive, got {(th, tw)}")
+    return th, tw
+
+
+def _equalize(dtype: np.dtype, entry: str, image: Any, nbins: Any, tile_shape: Any = None) -> Any:
     nb = _check_nbins(nbins)
+    tile = () if tile_shape is None else _check_tile_shape(tile_shape)
     fn = getattr(_native.load(), entry)
     if _is_torch(image):
         import torch
@@ -242,20 +258,20 @@ def _equalize(dtype: np.dtype, entry: str, image: Any, nbins: Any) -> Any:
         if image.dim() != 2:
             raise TypeError(f"argument 'image': dimensionality mismatch: expected 2, got {image.dim()}")
         if image.device.type == "cpu":
-            return torch.from_numpy(_equalize(dtype, entry, image.numpy(), nb))
+            return torch.from_numpy(_equalize(dtype, entry, image.numpy(), nb, tile_shape))
         ny, nx = image.shape
         with torch.cuda.device(image.device):
             out = torch.empty((ny, nx), dtype=tdtype, device=image.device)
             stream = torch.cuda.current_stream(image.device).cuda_stream
-            status = fn(image.data_ptr(), image.stride(0), image.stride(1), ny, nx, nb,
+            status = fn(image.data_ptr(), image.stride(0), image.stride(1), ny, nx, nb, *tile,
                         out.data_ptr(), _native.RLIC_DEVICE, ctypes.c_void_p(stream))
         _native.raise_for_status(status)
         return out
     img, st = _numpy_view(image, "image", dtype, 2)
     ny, nx = img.shape
     out = np.empty((ny, nx), dtype=dtype)
-    status = fn(img.ctypes.data, st[0], st[1], ny, nx, nb, out.ctypes.data, _native.RLIC_HOST,
-                None)
+    status = fn(img.ctypes.data, st[0], st[1], ny, nx, nb, *tile, out.ctypes.data,
+                _native.RLIC_HOST, None)
     _native.raise_for_status(status)
     return out
 
@@ -270,3 +286,29 @@ def equalize_histogram_f64(image, nbins):
     """float64 global histogram equalization (declared, unimplemented, in
     the reference: ``src/rlic/_core.pyi:34-37``)."""
     return _equalize(np.dtype("float64"), "rlic_equalize_histogram_f64", image, nbins)
+
+
+def equalize_histogram_sliding_tile_f32(image, nbins, tile_shape):
+    """float32 sliding-window adaptive equalization (declared, unimplemented,
+    in the reference: ``src/rlic/_core.pyi:38-42``)."""
+    return _equalize(np.dtype("float32"), "rlic_equalize_histogram_sliding_tile_f32", image,
+                     nbins, tile_shape)
+
+
+def equalize_histogram_sliding_tile_f64(image, nbins, tile_shape):
+    """float64 sliding-window adaptive equalization (``src/rlic/_core.pyi:43-47``)."""
+    return _equalize(np.dtype("float64"), "rlic_equalize_histogram_sliding_tile_f64", image,
+                     nbins, tile_shape)
+
+
+def equalize_histogram_tile_interpolation_f32(image, nbins, tile_shape):
+    """float32 tile-interpolated adaptive equalization (declared,
+    unimplemented, in the reference: ``src/rlic/_core.pyi:48-52``)."""
+    return _equalize(np.dtype("float32"), "rlic_equalize_histogram_tile_interpolation_f32", image,
+                     nbins, tile_shape)
+
+
+def equalize_histogram_tile_interpolation_f64(image, nbins, tile_shape):
+    """float64 tile-interpolated adaptive equalization (``src/rlic/_core.pyi:53-57``)."""
+    return _equalize(np.dtype("float64"), "rlic_equalize_histogram_tile_interpolation_f64", image,
+                     nbins, tile_shape)

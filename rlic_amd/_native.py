@@ -95,6 +95,14 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "rlic_convolve_noalloc_f64": (_CONVOLVE_ARGTYPES[:20] + _CONVOLVE_ARGTYPES[21:], _int),
     "rlic_equalize_histogram_f32": ([_vp, _i64, _i64, _i64, _i64, _i64, _vp, _int, _vp], _int),
     "rlic_equalize_histogram_f64": ([_vp, _i64, _i64, _i64, _i64, _i64, _vp, _int, _vp], _int),
+    "rlic_equalize_histogram_sliding_tile_f32":
+        ([_vp, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _vp, _int, _vp], _int),
+    "rlic_equalize_histogram_sliding_tile_f64":
+        ([_vp, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _vp, _int, _vp], _int),
+    "rlic_equalize_histogram_tile_interpolation_f32":
+        ([_vp, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _vp, _int, _vp], _int),
+    "rlic_equalize_histogram_tile_interpolation_f64":
+        ([_vp, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _vp, _int, _vp], _int),
     "rlic_last_error": ([], ctypes.c_char_p),
     "rlic_trace_stats_read": ([_i64p, _i64p], _int),
     "rlic_autotune_read": ([ctypes.c_char_p, _i64], _i64),

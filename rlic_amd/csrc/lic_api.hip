@@ -967,14 +967,25 @@ int convolve_noalloc_entry(const T* tex, int64_t tsy, int64_t tsx, const T* u, i
 // Largest bin count of an equalization (the counters are 4 bytes each).
 constexpr int64_t kMaxBins = int64_t{1} << 24;
 
+// Largest counter array of a tile-interpolated equalization (tiles x nbins
+// 4-byte counters).
+constexpr int64_t kMaxTileCounters = int64_t{1} << 30;
+
+// method 0: global; rlic::kEqSlidingTile / kEqTileInterpolation with a
+// tile_h x tile_w window or tile.
 template <typename T>
 int equalize_entry(const T* image, int64_t sy, int64_t sx, int64_t ny, int64_t nx, int64_t nbins,
-                   T* out, int location, void* stream_ptr) {
+                   T* out, int location, void* stream_ptr, int method = 0, int64_t tile_h = 1,
+                   int64_t tile_w = 1) {
   t_last_error.clear();
   if (ny < 0 || nx < 0) return set_error(RLIC_ERR_INVALID_ARGUMENT, "negative image size");
   if (nbins < 1 || nbins > kMaxBins)
     return set_error(RLIC_ERR_INVALID_ARGUMENT, "nbins must be in [1, %lld], got %lld",
                      static_cast<long long>(kMaxBins), static_cast<long long>(nbins));
+  if (method != 0 && (tile_h < 1 || tile_w < 1 || tile_h > INT32_MAX || tile_w > INT32_MAX))
+    return set_error(RLIC_ERR_INVALID_ARGUMENT,
+                     "tile_shape must be positive and below 2^31, got (%lld, %lld)",
+                     static_cast<long long>(tile_h), static_cast<long long>(tile_w));
   if (location != RLIC_HOST && location != RLIC_DEVICE)
     return set_error(RLIC_ERR_INVALID_ARGUMENT, "unknown memory location %d", location);
   if (ny == 0 || nx == 0) return RLIC_OK;  // an empty image: nothing to do
@@ -982,6 +993,15 @@ int equalize_entry(const T* image, int64_t sy, int64_t sx, int64_t ny, int64_t n
     return set_error(RLIC_ERR_INVALID_ARGUMENT,
                      "image of %lld x %lld pixels exceeds the supported 2^31-1 pixels",
                      static_cast<long long>(ny), static_cast<long long>(nx));
+  if (method == rlic::kEqTileInterpolation) {
+    const int64_t tiles = ((ny + std::min(tile_h, ny) - 1) / std::min(tile_h, ny)) *
+                          ((nx + std::min(tile_w, nx) - 1) / std::min(tile_w, nx));
+    if (tiles > kMaxTileCounters / nbins)
+      return set_error(RLIC_ERR_INVALID_ARGUMENT,
+                       "%lld tiles x %lld bins exceed the %lld tile counters supported",
+                       static_cast<long long>(tiles), static_cast<long long>(nbins),
+                       static_cast<long long>(kMaxTileCounters));
+  }
   if (!image || !out) return set_error(RLIC_ERR_INVALID_ARGUMENT, "null array pointer");
   int dev = 0;
   RLIC_HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
@@ -991,12 +1011,29 @@ int equalize_entry(const T* image, int64_t sy, int64_t sx, int64_t ny, int64_t n
   const int32_t bins = static_cast<int32_t>(nbins);
   auto run = [&](const T* img, int64_t isy, int64_t isx, T* dst) -> int {
     rlic::CallScratch scratch;
-    RLIC_HIP_TRY(scratch.get(rlic::equalize_scratch_bytes<T>(bins), stream),
+    const int32_t h = static_cast<int32_t>(ny), w = static_cast<int32_t>(nx);
+    const int32_t th = static_cast<int32_t>(tile_h), tw = static_cast<int32_t>(tile_w);
+    if (method == 0) {
+      RLIC_HIP_TRY(scratch.get(rlic::equalize_scratch_bytes<T>(bins), stream),
+                   "allocating equalization scratch");
+      LaunchTimer timer("lic_equalize", stream);
+      RLIC_HIP_TRY(rlic::launch_equalize<T>(img, isy, isx, h, w, bins, dst, scratch.ptr(), stream),
+                   "equalize");
+      return RLIC_OK;
+    }
+    RLIC_HIP_TRY(scratch.get(rlic::equalize_tiles_scratch_bytes<T>(method, h, w, bins, th, tw),
+                             stream),
                  "allocating equalization scratch");
-    LaunchTimer timer("lic_equalize", stream);
-    RLIC_HIP_TRY(rlic::launch_equalize<T>(img, isy, isx, static_cast<int32_t>(ny),
-                                          static_cast<int32_t>(nx), bins, dst, scratch.ptr(),
-                                          stream),
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) {
+      (void)hipGetLastError();
+      lds = 65536;
+    }
+    LaunchTimer timer(method == rlic::kEqSlidingTile ? "lic_equalize_sliding_tile"
+                                                     : "lic_equalize_tile_interpolation",
+                      stream);
+    RLIC_HIP_TRY(rlic::launch_equalize_tiles<T>(method, img, isy, isx, h, w, bins, th, tw, dst,
+                                                scratch.ptr(), lds, stream),
                  "equalize");
     return RLIC_OK;
   };
@@ -1180,6 +1217,38 @@ int rlic_equalize_histogram_f64(const double* image, int64_t sy, int64_t sx, int
                                 int64_t nx, int64_t nbins, double* out, int location,
                                 void* stream) {
   return equalize_entry<double>(image, sy, sx, ny, nx, nbins, out, location, stream);
+}
+
+int rlic_equalize_histogram_sliding_tile_f32(const float* image, int64_t sy, int64_t sx,
+                                             int64_t ny, int64_t nx, int64_t nbins,
+                                             int64_t tile_h, int64_t tile_w, float* out,
+                                             int location, void* stream) {
+  return equalize_entry<float>(image, sy, sx, ny, nx, nbins, out, location, stream,
+                               rlic::kEqSlidingTile, tile_h, tile_w);
+}
+
+int rlic_equalize_histogram_sliding_tile_f64(const double* image, int64_t sy, int64_t sx,
+                                             int64_t ny, int64_t nx, int64_t nbins,
+                                             int64_t tile_h, int64_t tile_w, double* out,
+                                             int location, void* stream) {
+  return equalize_entry<double>(image, sy, sx, ny, nx, nbins, out, location, stream,
+                                rlic::kEqSlidingTile, tile_h, tile_w);
+}
+
+int rlic_equalize_histogram_tile_interpolation_f32(const float* image, int64_t sy, int64_t sx,
+                                                   int64_t ny, int64_t nx, int64_t nbins,
+                                                   int64_t tile_h, int64_t tile_w, float* out,
+                                                   int location, void* stream) {
+  return equalize_entry<float>(image, sy, sx, ny, nx, nbins, out, location, stream,
+                               rlic::kEqTileInterpolation, tile_h, tile_w);
+}
+
+int rlic_equalize_histogram_tile_interpolation_f64(const double* image, int64_t sy, int64_t sx,
+                                                   int64_t ny, int64_t nx, int64_t nbins,
+                                                   int64_t tile_h, int64_t tile_w, double* out,
+                                                   int location, void* stream) {
+  return equalize_entry<double>(image, sy, sx, ny, nx, nbins, out, location, stream,
+                                rlic::kEqTileInterpolation, tile_h, tile_w);
 }
 
 int rlic_convolve_batch_f32(const rlic_field_f32* fields, int64_t n_fields, int profile,

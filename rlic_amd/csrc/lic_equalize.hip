@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "lic_equalize.h"
@@ -194,9 +195,11 @@ __global__ __launch_bounds__(kThreads) void lic_eq_hist(const T* __restrict__ im
   }
 }
 
-// Inclusive scan of the counts, in place (one block of kScanThreads).
+// Inclusive scan of the counts, in place: block k scans the k-th array of
+// nbins counters (one block for the global histogram, one per tile).
 constexpr int kScanThreads = 1024;
-__global__ __launch_bounds__(kScanThreads) void lic_eq_scan(uint32_t* hist, int32_t nbins) {
+__global__ __launch_bounds__(kScanThreads) void lic_eq_scan(uint32_t* hist_all, int32_t nbins) {
+  uint32_t* hist = hist_all + static_cast<int64_t>(blockIdx.x) * nbins;
   __shared__ uint32_t part[kScanThreads / 64];
   __shared__ uint32_t carry;
   if (threadIdx.x == 0) carry = 0u;
@@ -234,6 +237,199 @@ __global__ __launch_bounds__(kThreads) void lic_eq_map(const T* __restrict__ img
     if (__builtin_isfinite(x)) {
       const int32_t b = r.constant ? 0 : eq_bin(static_cast<double>(x), r, nbins);
       y = static_cast<T>(static_cast<double>(cdf[b]) / static_cast<double>(r.count));
+    }
+    out[p] = y;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Adaptive equalization (the reference's sliding-tile and tile-interpolation
+// stubs, src/rlic/_core.pyi:38-57; semantics in oracle/equalize.py): the
+// global bins above, counted per window or per tile.
+// ---------------------------------------------------------------------------
+
+// Bin of every pixel (-1 where not finite), C order.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void lic_eq_bins(const T* __restrict__ img, int64_t n,
+                                                       int32_t nx, int64_t sy, int64_t sx,
+                                                       int32_t nbins, const EqState<T>* st,
+                                                       int32_t* __restrict__ bins) {
+  const EqRange r = read_range(st, nbins);
+  for (int64_t p = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; p < n;
+       p += static_cast<int64_t>(gridDim.x) * kThreads) {
+    const T x = load_px(img, p, nx, sy, sx);
+    bins[p] = __builtin_isfinite(x) ? (r.constant ? 0 : eq_bin(static_cast<double>(x), r, nbins))
+                                    : -1;
+  }
+}
+
+// Sliding tile: out = (finite window pixels with a bin <= the pixel's) /
+// (finite window pixels).  A block covers kSlide x kSlide pixels; with LDS
+// it first stages the bins of its pixels' windows (the block plus th - 1
+// rows and tw - 1 columns, -1 outside the image) as dynamic LDS.  A bin
+// compares as unsigned, so -1 (not finite) is never "<=" a pixel's bin.
+constexpr int32_t kSlide = 16;
+
+template <typename T, bool LDS>
+__global__ __launch_bounds__(kSlide * kSlide) void lic_eq_sliding(
+    const T* __restrict__ img, int32_t ny, int32_t nx, int64_t sy, int64_t sx,
+    const int32_t* __restrict__ bins, int32_t th, int32_t tw, T* __restrict__ out) {
+  extern __shared__ int32_t box[];
+  const int32_t up = (th - 1) / 2, left = (tw - 1) / 2;
+  const int32_t i0 = blockIdx.y * kSlide, j0 = blockIdx.x * kSlide;
+  const int32_t bw = kSlide + tw - 1;
+  if (LDS) {
+    const int32_t bh = kSlide + th - 1;
+    for (int32_t k = threadIdx.x; k < bh * bw; k += kSlide * kSlide) {
+      const int32_t r = k / bw, c = k - r * bw;
+      const int32_t i = i0 - up + r, j = j0 - left + c;
+      box[k] = (i >= 0 && i < ny && j >= 0 && j < nx) ? bins[static_cast<int64_t>(i) * nx + j]
+                                                        : -1;
+    }
+    __syncthreads();
+  }
+  const int32_t ti = threadIdx.x / kSlide, tj = threadIdx.x % kSlide;
+  const int32_t i = i0 + ti, j = j0 + tj;
+  if (i >= ny || j >= nx) return;
+  const int64_t p = static_cast<int64_t>(i) * nx + j;
+  const T x = img[i * sy + j * sx];
+  const int32_t bp = bins[p];
+  if (bp < 0) {
+    out[p] = x;
+    return;
+  }
+  uint32_t below = 0, count = 0;
+  if (LDS) {
+    for (int32_t r = 0; r < th; ++r) {
+      const int32_t* row = box + (ti + r) * bw + tj;
+      for (int32_t c = 0; c < tw; ++c) {
+        const int32_t q = row[c];
+        count += q >= 0 ? 1u : 0u;
+        below += static_cast<uint32_t>(q) <= static_cast<uint32_t>(bp) ? 1u : 0u;
+      }
+    }
+  } else {
+    const int32_t r0 = max(0, i - up), r1 = min(ny, i - up + th);
+    const int32_t c0 = max(0, j - left), c1 = min(nx, j - left + tw);
+    for (int32_t r = r0; r < r1; ++r) {
+      const int32_t* row = bins + static_cast<int64_t>(r) * nx;
+      for (int32_t c = c0; c < c1; ++c) {
+        const int32_t q = row[c];
+        count += q >= 0 ? 1u : 0u;
+        below += static_cast<uint32_t>(q) <= static_cast<uint32_t>(bp) ? 1u : 0u;
+      }
+    }
+  }
+  out[p] = static_cast<T>(static_cast<double>(below) / static_cast<double>(count));
+}
+
+// Tile interpolation.  Tile t = ty * tiles_x + tx covers rows
+// [ty * th, min((ty + 1) * th, ny)) and columns alike; its counts go to
+// hist[t * nbins ..].  With LDS a block counts its tile in LDS and writes
+// every bin; without, the counters were cleared first and it adds with
+// global atomics.
+struct TileGrid {
+  int32_t th, tw, tiles_y, tiles_x;
+};
+
+template <typename T, bool LDS>
+__global__ __launch_bounds__(kThreads) void lic_eq_tile_hist(
+    const T* __restrict__ img, int32_t ny, int32_t nx, int64_t sy, int64_t sx, int32_t nbins,
+    const EqState<T>* st, TileGrid tg, uint32_t* __restrict__ hist_all) {
+  __shared__ uint32_t local[LDS ? kLdsBins : 1];
+  const EqRange r = read_range(st, nbins);
+  const int32_t t = blockIdx.x;
+  const int32_t ty = t / tg.tiles_x, tx = t - ty * tg.tiles_x;
+  uint32_t* hist = hist_all + static_cast<int64_t>(t) * nbins;
+  if (LDS) {
+    for (int32_t b = threadIdx.x; b < nbins; b += kThreads) local[b] = 0u;
+    __syncthreads();
+  }
+  const int32_t i0 = ty * tg.th, j0 = tx * tg.tw;
+  const int32_t hh = min(tg.th, ny - i0), ww = min(tg.tw, nx - j0);
+  const int64_t cells = static_cast<int64_t>(hh) * ww;
+  for (int64_t k = threadIdx.x; k < cells; k += kThreads) {
+    const int64_t a = k / ww;
+    const int64_t i = i0 + a, j = j0 + (k - a * ww);
+    const T x = img[i * sy + j * sx];
+    if (!__builtin_isfinite(x)) continue;
+    const int32_t b = r.constant ? 0 : eq_bin(static_cast<double>(x), r, nbins);
+    if (LDS) {
+      atomicAdd(&local[b], 1u);
+    } else {
+      atomicAdd(&hist[b], 1u);
+    }
+  }
+  if (LDS) {
+    __syncthreads();
+    for (int32_t b = threadIdx.x; b < nbins; b += kThreads) hist[b] = local[b];
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void lic_eq_zero(uint32_t* __restrict__ p, int64_t n) {
+  for (int64_t k = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; k < n;
+       k += static_cast<int64_t>(gridDim.x) * kThreads)
+    p[k] = 0u;
+}
+
+// Tiles k, k + 1 and the weight of k + 1 for index i along an axis of n
+// pixels cut into tiles of t (centre of tile k: k t + (extent - 1) / 2).
+__device__ __forceinline__ double tile_centre(int32_t k, int32_t t, int32_t n) {
+  const int32_t extent = min(t, n - k * t);
+  return static_cast<double>(static_cast<int64_t>(k) * t) + static_cast<double>(extent - 1) * 0.5;
+}
+
+__device__ __forceinline__ void axis_weights(int32_t i, int32_t n, int32_t t, int32_t tiles,
+                                             int32_t& a0, int32_t& a1, double& w) {
+  const double x = static_cast<double>(i);
+  a0 = a1 = 0;
+  w = 0.0;
+  if (tiles == 1 || x <= tile_centre(0, t, n)) return;
+  if (x >= tile_centre(tiles - 1, t, n)) {
+    a0 = a1 = tiles - 1;
+    return;
+  }
+  // the last k with centre(k) <= x, in [0, tiles - 2]: a first guess from
+  // full tiles, then corrected (the last tile may be partial)
+  int32_t k = static_cast<int32_t>((x - static_cast<double>(t - 1) * 0.5) / t);
+  k = k < 0 ? 0 : (k > tiles - 2 ? tiles - 2 : k);
+  while (k < tiles - 2 && tile_centre(k + 1, t, n) <= x) ++k;
+  while (k > 0 && tile_centre(k, t, n) > x) --k;
+  a0 = k;
+  a1 = k + 1;
+  const double c0 = tile_centre(k, t, n);
+  w = (x - c0) / (tile_centre(k + 1, t, n) - c0);
+}
+
+__device__ __forceinline__ double lerp_d(double a, double b, double w) { return a + w * (b - a); }
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void lic_eq_tile_map(
+    const T* __restrict__ img, int32_t ny, int32_t nx, int64_t sy, int64_t sx, int32_t nbins,
+    const EqState<T>* st, const uint32_t* __restrict__ gcdf, const uint32_t* __restrict__ tcdf,
+    TileGrid tg, T* __restrict__ out) {
+  const EqRange r = read_range(st, nbins);
+  const int64_t n = static_cast<int64_t>(ny) * nx;
+  for (int64_t p = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; p < n;
+       p += static_cast<int64_t>(gridDim.x) * kThreads) {
+    const int32_t i = static_cast<int32_t>(p / nx), j = static_cast<int32_t>(p - int64_t{i} * nx);
+    const T x = img[i * sy + j * sx];
+    T y = x;
+    if (__builtin_isfinite(x)) {
+      const int32_t b = r.constant ? 0 : eq_bin(static_cast<double>(x), r, nbins);
+      int32_t y0, y1, x0, x1;
+      double wy, wx;
+      axis_weights(i, ny, tg.th, tg.tiles_y, y0, y1, wy);
+      axis_weights(j, nx, tg.tw, tg.tiles_x, x0, x1, wx);
+      auto map = [&](int32_t ty, int32_t tx) {
+        const uint32_t* c = tcdf + (static_cast<int64_t>(ty) * tg.tiles_x + tx) * nbins;
+        const uint32_t total = c[nbins - 1];
+        return total != 0u ? static_cast<double>(c[b]) / static_cast<double>(total)
+                           : static_cast<double>(gcdf[b]) / static_cast<double>(r.count);
+      };
+      const double top = lerp_d(map(y0, x0), map(y0, x1), wx);
+      const double bot = lerp_d(map(y1, x0), map(y1, x1), wx);
+      y = static_cast<T>(lerp_d(top, bot, wy));
     }
     out[p] = y;
   }
@@ -288,6 +484,98 @@ hipError_t launch_equalize(const T* img, int64_t sy, int64_t sx, int32_t ny, int
   return hipGetLastError();
 }
 
+int32_t tiles_along(int32_t n, int32_t t) { return (n + t - 1) / t; }
+
+template <typename T>
+size_t equalize_tiles_scratch_bytes(int method, int32_t ny, int32_t nx, int32_t nbins,
+                                    int32_t th, int32_t tw) {
+  size_t b = equalize_scratch_bytes<T>(nbins);
+  b = (b + 255) & ~size_t{255};
+  if (method == kEqSlidingTile)
+    return b + sizeof(int32_t) * static_cast<size_t>(ny) * static_cast<size_t>(nx);
+  const size_t tiles = static_cast<size_t>(tiles_along(ny, th)) * tiles_along(nx, tw);
+  return b + sizeof(uint32_t) * tiles * static_cast<size_t>(nbins);
+}
+
+template <typename T>
+hipError_t launch_equalize_tiles(int method, const T* img, int64_t sy, int64_t sx, int32_t ny,
+                                 int32_t nx, int32_t nbins, int32_t th, int32_t tw, T* out,
+                                 void* scratch, int32_t lds_limit, hipStream_t stream) {
+  const int64_t n = static_cast<int64_t>(ny) * nx;
+  auto* st = static_cast<EqState<T>*>(scratch);
+  auto* hist = reinterpret_cast<uint32_t*>(static_cast<char*>(scratch) +
+                                           ((sizeof(EqState<T>) + 255) & ~size_t{255}));
+  char* extra = static_cast<char*>(scratch) +
+                ((equalize_scratch_bytes<T>(nbins) + 255) & ~size_t{255});
+  const int init_blocks = static_cast<int>((nbins + kThreads - 1) / kThreads > 1024
+                                               ? 1024
+                                               : (nbins + kThreads - 1) / kThreads);
+  hipLaunchKernelGGL(lic_eq_init<T>, dim3(init_blocks < 1 ? 1 : init_blocks), dim3(kThreads), 0,
+                     stream, st, hist, nbins);
+  const int g = grid_for(n, 8);
+  hipLaunchKernelGGL(lic_eq_range<T>, dim3(g), dim3(kThreads), 0, stream, img, n, nx, sy, sx, st);
+  if (method == kEqSlidingTile) {
+    // a window taller than 2 ny - 1 rows covers the image from every pixel,
+    // as one of exactly 2 ny - 1 does (likewise for columns)
+    th = static_cast<int32_t>(std::min<int64_t>(th, 2 * int64_t{ny} - 1));
+    tw = static_cast<int32_t>(std::min<int64_t>(tw, 2 * int64_t{nx} - 1));
+    auto* bins = reinterpret_cast<int32_t*>(extra);
+    hipLaunchKernelGGL(lic_eq_bins<T>, dim3(g), dim3(kThreads), 0, stream, img, n, nx, sy, sx,
+                       nbins, st, bins);
+    const dim3 grid(static_cast<unsigned>((nx + kSlide - 1) / kSlide),
+                    static_cast<unsigned>((ny + kSlide - 1) / kSlide));
+    const int64_t box = int64_t{kSlide + th - 1} * (kSlide + tw - 1) * int64_t{sizeof(int32_t)};
+    if (box <= lds_limit) {
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&lic_eq_sliding<T, true>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(box));
+      (void)hipGetLastError();  // advisory on this target
+      hipLaunchKernelGGL((lic_eq_sliding<T, true>), grid, dim3(kSlide * kSlide),
+                         static_cast<unsigned>(box), stream, img, ny, nx, sy, sx, bins, th, tw,
+                         out);
+    } else {
+      hipLaunchKernelGGL((lic_eq_sliding<T, false>), grid, dim3(kSlide * kSlide), 0, stream, img,
+                         ny, nx, sy, sx, bins, th, tw, out);
+    }
+    return hipGetLastError();
+  }
+  // tile interpolation: the global histogram (for tiles without a finite
+  // pixel), the tiles' histograms, their cumulative sums, the map
+  const TileGrid tg{std::min(th, ny), std::min(tw, nx), tiles_along(ny, std::min(th, ny)),
+                    tiles_along(nx, std::min(tw, nx))};
+  const int32_t tiles = tg.tiles_y * tg.tiles_x;
+  auto* tcdf = reinterpret_cast<uint32_t*>(extra);
+  const int gh = grid_for(n, nbins <= 1024 ? 8 : 2);
+  if (nbins <= kLdsBins) {
+    hipLaunchKernelGGL((lic_eq_hist<T, true>), dim3(gh), dim3(kThreads), 0, stream, img, n, nx,
+                       sy, sx, nbins, st, hist);
+    hipLaunchKernelGGL((lic_eq_tile_hist<T, true>), dim3(tiles), dim3(kThreads), 0, stream, img,
+                       ny, nx, sy, sx, nbins, st, tg, tcdf);
+  } else {
+    hipLaunchKernelGGL((lic_eq_hist<T, false>), dim3(gh), dim3(kThreads), 0, stream, img, n, nx,
+                       sy, sx, nbins, st, hist);
+    const int64_t counters = static_cast<int64_t>(tiles) * nbins;
+    hipLaunchKernelGGL(lic_eq_zero, dim3(grid_for(counters, 8)), dim3(kThreads), 0, stream, tcdf,
+                       counters);
+    hipLaunchKernelGGL((lic_eq_tile_hist<T, false>), dim3(tiles), dim3(kThreads), 0, stream, img,
+                       ny, nx, sy, sx, nbins, st, tg, tcdf);
+  }
+  hipLaunchKernelGGL(lic_eq_scan, dim3(1), dim3(kScanThreads), 0, stream, hist, nbins);
+  hipLaunchKernelGGL(lic_eq_scan, dim3(tiles), dim3(kScanThreads), 0, stream, tcdf, nbins);
+  hipLaunchKernelGGL(lic_eq_tile_map<T>, dim3(g), dim3(kThreads), 0, stream, img, ny, nx, sy, sx,
+                     nbins, st, hist, tcdf, tg, out);
+  return hipGetLastError();
+}
+
+template size_t equalize_tiles_scratch_bytes<float>(int, int32_t, int32_t, int32_t, int32_t,
+                                                    int32_t);
+template size_t equalize_tiles_scratch_bytes<double>(int, int32_t, int32_t, int32_t, int32_t,
+                                                     int32_t);
+template hipError_t launch_equalize_tiles<float>(int, const float*, int64_t, int64_t, int32_t,
+                                                 int32_t, int32_t, int32_t, int32_t, float*,
+                                                 void*, int32_t, hipStream_t);
+template hipError_t launch_equalize_tiles<double>(int, const double*, int64_t, int64_t, int32_t,
+                                                  int32_t, int32_t, int32_t, int32_t, double*,
+                                                  void*, int32_t, hipStream_t);
 template size_t equalize_scratch_bytes<float>(int32_t);
 template size_t equalize_scratch_bytes<double>(int32_t);
 template hipError_t launch_equalize<float>(const float*, int64_t, int64_t, int32_t, int32_t,

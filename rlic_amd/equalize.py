@@ -9,15 +9,24 @@ infinite pixels are copied unchanged.  A ROCm tensor, e.g. the output of
 stream without leaving HBM; NumPy arrays are uploaded and the result
 downloaded.
 
-The reference declares ``equalize_histogram_f32/_f64(image, nbins)`` in its
-native-module stub (``src/rlic/_core.pyi:30-37``) but has no implementation
-(post-processing is out of its scope, ``README.md:19-23``); these are the
-same names and arguments.
+``equalize_histogram_sliding_tile(image, nbins, tile_shape)`` equalizes
+each pixel against its own window of ``tile_shape`` pixels (centred, clipped
+to the image), and ``equalize_histogram_tile_interpolation(image, nbins,
+tile_shape)`` against the mappings of the tiles of ``tile_shape`` around
+it, interpolated bilinearly between tile centres; both bin over the whole
+image's finite range.
+
+The reference declares ``equalize_histogram_f32/_f64(image, nbins)`` and
+the ``_sliding_tile`` / ``_tile_interpolation`` variants with a
+``tile_shape`` in its native-module stub (``src/rlic/_core.pyi:30-57``) but
+has no implementation (post-processing is out of its scope,
+``README.md:19-23``); these are the same names and arguments.
 """
 
 from __future__ import annotations
 
-__all__ = ["equalize_histogram"]
+__all__ = ["equalize_histogram", "equalize_histogram_sliding_tile",
+           "equalize_histogram_tile_interpolation"]
 
 from typing import Any
 
@@ -37,11 +46,33 @@ def _dtype_of(image: Any):
     return None
 
 
-def equalize_histogram(image: Any, nbins: int = 256) -> Any:
-    """Global histogram equalization of a 2-D float32 or float64 image."""
+def _dtype_or_raise(image: Any):
     dtype = _dtype_of(image)
     if dtype is None:
         found = getattr(image, "dtype", type(image).__name__)
         raise TypeError(f"image must be a float32 or float64 array or tensor, got {found}")
+    return dtype
+
+
+def equalize_histogram(image: Any, nbins: int = 256) -> Any:
+    """Global histogram equalization of a 2-D float32 or float64 image."""
+    dtype = _dtype_or_raise(image)
     fn = _core.equalize_histogram_f32 if dtype == np.float32 else _core.equalize_histogram_f64
     return fn(image, nbins)
+
+
+def equalize_histogram_sliding_tile(image: Any, nbins: int, tile_shape) -> Any:
+    """Adaptive equalization: each pixel against its (th, tw) window."""
+    dtype = _dtype_or_raise(image)
+    fn = (_core.equalize_histogram_sliding_tile_f32 if dtype == np.float32
+          else _core.equalize_histogram_sliding_tile_f64)
+    return fn(image, nbins, tile_shape)
+
+
+def equalize_histogram_tile_interpolation(image: Any, nbins: int, tile_shape) -> Any:
+    """Adaptive equalization: per-tile mappings interpolated between tile
+    centres."""
+    dtype = _dtype_or_raise(image)
+    fn = (_core.equalize_histogram_tile_interpolation_f32 if dtype == np.float32
+          else _core.equalize_histogram_tile_interpolation_f64)
+    return fn(image, nbins, tile_shape)

@@ -197,3 +197,184 @@ def test_gpu_in_place_through_the_c_abi(gpu_device):
                                              _native.RLIC_DEVICE, ctypes.c_void_p(stream))
     assert status == 0, _native.last_error()
     assert np.array_equal(t.cpu().numpy(), eq.equalize_histogram(img, 100))
+
+
+# ---------------------------------------------------------------------------
+# Adaptive equalization: sliding tile and tile interpolation (the reference's
+# declared, unimplemented stubs, src/rlic/_core.pyi:38-57).
+# ---------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_sliding_tile_reduces_to_global_and_to_one(dtype):
+    rng = np.random.default_rng(11)
+    img = rng.random((13, 21)).astype(dtype)
+    img[4, 5] = np.nan
+    g = eq.equalize_histogram(img, 32)
+    # a window of 2n - 1 (or more) covers the whole image from every pixel
+    for shape in [(25, 41), (26, 60), (100, 100)]:
+        assert np.array_equal(eq.equalize_histogram_sliding_tile(img, 32, shape), g,
+                              equal_nan=True), shape
+    # a 1 x 1 window holds only the pixel itself
+    one = eq.equalize_histogram_sliding_tile(img, 32, (1, 1))
+    assert np.array_equal(one, np.where(np.isnan(img), img, dtype(1)), equal_nan=True)
+
+
+@pytest.mark.parametrize("shape", [(3, 3), (4, 5), (2, 7), (1, 6), (9, 2)])
+def test_sliding_tile_bruteforce(shape):
+    rng = np.random.default_rng(sum(shape))
+    img = rng.random((11, 9))
+    img[rng.random(img.shape) < 0.1] = np.nan
+    img[2, 2] = img[6, 7]
+    for nbins in (1, 5, 40):
+        brute = eq.equalize_histogram_sliding_tile_bruteforce(img, nbins, shape)
+        for method in ("shifted", "integral"):
+            assert np.array_equal(eq.equalize_histogram_sliding_tile(img, nbins, shape, method),
+                                  brute, equal_nan=True), (shape, nbins, method)
+
+
+def test_tile_interpolation_closed_forms():
+    rng = np.random.default_rng(12)
+    img = rng.random((10, 14))
+    g = eq.equalize_histogram(img, 16)
+    for shape in [(10, 14), (64, 64)]:  # one tile: the global mapping
+        assert np.array_equal(eq.equalize_histogram_tile_interpolation(img, 16, shape), g)
+    # two constant tiles side by side, 0 on the left and 1 on the right,
+    # nbins = 2: the left tile maps both bins to 1, the right one bin 0 to 0
+    # and bin 1 to 1.  Tile centres at columns 1.5 and 5.5: a left pixel in
+    # column j gets lerp(1, 0, w) = 1 - w, w = clamp((j - 1.5) / 4); right
+    # pixels get 1 everywhere.
+    two = np.zeros((3, 8))
+    two[:, 4:] = 1.0
+    out = eq.equalize_histogram_tile_interpolation(two, 2, (3, 4))
+    w = np.clip((np.arange(4) - 1.5) / 4.0, 0.0, None)
+    assert np.array_equal(out[:, :4], np.broadcast_to(1.0 - w, (3, 4)))
+    assert np.array_equal(out[:, 4:], np.ones((3, 4)))
+    # constant image: every mapping is 1
+    assert np.array_equal(eq.equalize_histogram_tile_interpolation(np.full((7, 9), 3.0), 8,
+                                                                   (2, 4)), np.ones((7, 9)))
+    # at the centre of a full odd tile the weights vanish: the tile's own
+    # equalization (with the global bins) of that pixel
+    img = rng.random((15, 15))
+    out = eq.equalize_histogram_tile_interpolation(img, 8, (5, 5))
+    b, _, _ = eq.bins(img, 8)
+    for ty in range(3):
+        for tx in range(3):
+            i, j = 5 * ty + 2, 5 * tx + 2
+            tile = b[5 * ty:5 * ty + 5, 5 * tx:5 * tx + 5]
+            assert out[i, j] == np.count_nonzero(tile <= b[i, j]) / 25.0
+
+
+def test_tile_interpolation_non_finite_and_empty_tiles():
+    img = np.random.default_rng(13).random((8, 8))
+    img[:4, :4] = np.nan  # a tile without finite pixels takes the global mapping
+    img[6, 6] = np.inf
+    out = eq.equalize_histogram_tile_interpolation(img, 4, (4, 4))
+    assert np.all(np.isnan(out[:4, :4])) and out[6, 6] == np.inf
+    assert np.all((out[np.isfinite(img)] > 0) & (out[np.isfinite(img)] <= 1))
+
+
+def test_tile_interpolation_planted_mistake_is_caught(monkeypatch):
+    # tile centres without the half-pixel shift: the two-tile closed form fails
+    two = np.zeros((3, 8))
+    two[:, 4:] = 1.0
+    real = eq._centres
+
+    def shifted(n, t):
+        return real(n, t) + 0.5
+
+    monkeypatch.setattr(eq, "_centres", shifted)
+    out = eq.equalize_histogram_tile_interpolation(two, 2, (3, 4))
+    monkeypatch.undo()
+    w = np.clip((np.arange(4) - 1.5) / 4.0, 0.0, None)
+    assert not np.array_equal(out[:, :4], np.broadcast_to(1.0 - w, (3, 4)))
+
+
+def test_adaptive_argument_errors():
+    for fn in (rlic_amd.equalize_histogram_sliding_tile,
+               rlic_amd.equalize_histogram_tile_interpolation):
+        with pytest.raises(ValueError, match="tile_shape"):
+            fn(np.ones((3, 3)), 4, (0, 2))
+        with pytest.raises(TypeError, match="tile_shape"):
+            fn(np.ones((3, 3)), 4, 3)
+        with pytest.raises(ValueError, match="nbins"):
+            fn(np.ones((3, 3)), 0, (2, 2))
+    lib = _native.load()
+    a = np.ones((2, 2))
+    for name in ("rlic_equalize_histogram_sliding_tile_f64",
+                 "rlic_equalize_histogram_tile_interpolation_f64"):
+        fn = getattr(lib, name)
+        assert fn(a.ctypes.data, 2, 1, 2, 2, 4, 0, 3, a.ctypes.data, 0, None) == \
+            _native.RLIC_ERR_INVALID_ARGUMENT
+        assert "tile_shape" in _native.last_error()
+        assert fn(a.ctypes.data, 2, 1, 0, 2, 4, 2, 2, a.ctypes.data, 0, None) == 0  # empty
+    big = np.ones((4096, 4096))
+    st = lib.rlic_equalize_histogram_tile_interpolation_f64(big.ctypes.data, 4096, 1, 4096, 4096,
+                                                            1 << 24, 8, 8, big.ctypes.data, 0,
+                                                            None)
+    assert st == _native.RLIC_ERR_INVALID_ARGUMENT and "tile counters" in _native.last_error()
+
+
+_ADAPTIVE_SHAPES = [(1, 1), (5, 5), (8, 3), (31, 17), (64, 64), (400, 7), (301, 301)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("nbins", [1, 16, 256, 20000])
+def test_gpu_sliding_tile_matches_oracle(gpu_device, dtype, nbins):
+    """Windows from 1 x 1 to larger than the images (301 x 301: its staged
+    bins exceed the LDS, so the window is counted from global memory)."""
+    rng = np.random.default_rng(nbins + 100)
+    for name, img, t in _images(rng, dtype, gpu_device):
+        for shape in _ADAPTIVE_SHAPES:
+            if nbins == 20000 and shape[0] * shape[1] > 64:
+                continue  # many distinct bins under a big window: a slow oracle, same kernels
+            exp = eq.equalize_histogram_sliding_tile(img, nbins, shape)
+            got = rlic_amd.equalize_histogram_sliding_tile(img, nbins, shape)
+            assert np.array_equal(got, exp, equal_nan=True), (name, shape, nbins)
+            dev = rlic_amd.equalize_histogram_sliding_tile(t, nbins, shape)
+            assert np.array_equal(dev.cpu().numpy(), exp, equal_nan=True), (name, shape, "device")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("nbins", [1, 16, 256, 20000])
+def test_gpu_tile_interpolation_matches_oracle(gpu_device, dtype, nbins):
+    """Tiles from 1 x 1 (every pixel its own tile) to larger than the
+    images; 20000 bins take the global-atomic tile histograms."""
+    rng = np.random.default_rng(nbins + 200)
+    for name, img, t in _images(rng, dtype, gpu_device):
+        for shape in _ADAPTIVE_SHAPES:
+            if nbins == 20000 and shape in ((1, 1), (5, 5), (8, 3)):
+                continue  # tiles x bins beyond what the oracle builds quickly
+            exp = eq.equalize_histogram_tile_interpolation(img, nbins, shape)
+            got = rlic_amd.equalize_histogram_tile_interpolation(img, nbins, shape)
+            assert np.array_equal(got, exp, equal_nan=True), (name, shape, nbins)
+            dev = rlic_amd.equalize_histogram_tile_interpolation(t, nbins, shape)
+            assert np.array_equal(dev.cpu().numpy(), exp, equal_nan=True), (name, shape, "device")
+
+
+@pytest.mark.gpu
+def test_gpu_adaptive_on_convolve_output_in_place(gpu_device):
+    """A device LIC image equalized in place by both adaptive kernels
+    through the C ABI, against the oracle on the same image."""
+    import torch
+
+    rng = np.random.default_rng(14)
+    shape = (257, 383)
+    t = [torch.from_numpy(a).to(gpu_device) for a in (
+        rng.random(shape).astype(np.float32), (rng.random(shape) - 0.5).astype(np.float32),
+        (rng.random(shape) - 0.5).astype(np.float32), np.ones(31, dtype=np.float32))]
+    lic = rlic_amd.convolve(t[0], t[1], t[2], kernel=t[3], boundaries="closed", iterations=2)
+    host = lic.cpu().numpy()
+    lib = _native.load()
+    stream = ctypes.c_void_p(torch.cuda.current_stream(gpu_device).cuda_stream)
+    for name, oracle, tile in (("sliding_tile", eq.equalize_histogram_sliding_tile, (33, 65)),
+                               ("tile_interpolation", eq.equalize_histogram_tile_interpolation,
+                                (64, 48))):
+        work = lic.clone()
+        fn = getattr(lib, f"rlic_equalize_histogram_{name}_f32")
+        status = fn(work.data_ptr(), shape[1], 1, shape[0], shape[1], 256, tile[0], tile[1],
+                    work.data_ptr(), _native.RLIC_DEVICE, stream)
+        assert status == 0, _native.last_error()
+        assert np.array_equal(work.cpu().numpy(), oracle(host, 256, tile)), name
