@@ -401,6 +401,67 @@ def single_call_measure(wl, dev, steps: int, warmup: int) -> dict:
             "inputs": wl.meta.get("generated_on", "host") + "-generated"}
 
 
+def graph_call_measure(wl, dev, steps: int, warmup: int) -> dict:
+    """One configuration as a whole allocation-free call
+    (rlic_convolve_noalloc_*) captured in one HIP graph and replayed:
+    microseconds per replay (synchronized start and end over `steps`
+    replays), beside the same call launched directly; the replay's output
+    is checked against the direct call's."""
+    import ctypes
+
+    import torch
+
+    from rlic_amd import _native
+    from rlic_amd._boundaries import BoundarySet
+
+    lib = _native.load()
+    t = [torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+         for a in (wl.texture, wl.u, wl.v, wl.kernel)]
+    f32 = t[0].dtype == torch.float32
+    fn = lib.rlic_convolve_noalloc_f32 if f32 else lib.rlic_convolve_noalloc_f64
+    bs = BoundarySet.from_spec(wl.boundaries)
+    code = {"closed": 0, "periodic": 1}
+    bounds = [code[bs.x[0]], code[bs.x[1]], code[bs.y[0]], code[bs.y[1]]]
+    ny, nx = wl.shape
+    out = torch.empty_like(t[0])
+    direct = torch.empty_like(t[0])
+
+    def call(dst, stream):
+        st = fn(t[0].data_ptr(), nx, 1, t[1].data_ptr(), nx, 1, t[2].data_ptr(), nx, 1, ny, nx,
+                t[3].data_ptr(), t[3].shape[0], {"velocity": 0, "polarization": 1}[wl.uv_mode],
+                *bounds, wl.iterations, dst.data_ptr(), 3, ctypes.c_void_p(stream.cuda_stream))
+        _native.raise_for_status(st)
+
+    side = torch.cuda.Stream(dev)
+    side.wait_stream(torch.cuda.current_stream(dev))
+    with torch.cuda.stream(side):
+        call(direct, side)
+    side.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=side):
+        call(out, torch.cuda.current_stream(dev))
+
+    def timed(step):
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps * 1e6
+
+    stream = torch.cuda.current_stream(dev)
+    us_direct = timed(lambda: call(direct, stream))
+    us_graph = timed(graph.replay)
+    same = bool(torch.equal(out, direct))
+    return {"workload": wl.name, "iterations": wl.iterations,
+            "us_per_call_direct": round(us_direct, 2), "us_per_replay": round(us_graph, 2),
+            "value": round(wl.pixel_steps / (us_graph / 1e6) / 1e9, 3),
+            "unit": "Gpixel-steps/s", "same_bits_as_direct": same,
+            "steps": steps, "warmup": warmup}
+
+
 def extras_measure(dev, args) -> dict:
     """The other configurations, each on its own inputs (freed after)."""
     import torch
@@ -410,6 +471,8 @@ def extras_measure(dev, args) -> dict:
     out = {}
     log("bench: extra readme_example (256^2 float64, L=65)")
     out["readme_256_f64"] = single_call_measure(readme_example(), dev, steps=50, warmup=5)
+    log("bench: extra readme_example as one captured graph per call")
+    out["readme_256_f64_graph"] = graph_call_measure(readme_example(), dev, steps=200, warmup=10)
     for key, n, dtype in (("polarization_16384_f32", 16384, np.float32),
                           ("polarization_8192_f64", 8192, np.float64)):
         log(f"bench: extra {key} (inputs generated on the device)")
