@@ -220,14 +220,17 @@ def test_background_loop_with_concurrent_submitters(gpu_device):
 
 
 @pytest.mark.gpu
-def test_large_fields_replay_as_single_launches(gpu_device):
+@pytest.mark.parametrize("streams", ["1", "4"])
+def test_large_fields_replay_as_single_launches(gpu_device, monkeypatch, streams):
     """Replay passes of fields of at least 2^18 pixels run as single-field
-    launches (LDS box or hybrid replay) inside the serving step, next to the
-    persistent launch of the small fields; every request, follow-ups
-    included, gives the bits of one convolve call."""
+    launches (LDS box or hybrid replay) inside the serving step, on the
+    server's stream or spread over side streams, next to the persistent
+    launch of the small fields; every request, follow-ups included, gives
+    the bits of one convolve call."""
     from rlic_amd import _native
     from rlic_amd.serve import Server
 
+    monkeypatch.setenv("RLIC_BATCH_STREAMS", streams)
     rng = np.random.default_rng(13)
     reqs = _requests(rng, 4)
     for k, (dtype, klen) in enumerate([(np.float32, 65), (np.float32, 201), (np.float64, 99)]):

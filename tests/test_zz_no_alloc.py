@@ -110,3 +110,53 @@ def test_noalloc_rejects_sizes_beyond_its_array():
     assert "unknown arithmetic profile" in _native.last_error()
     args[9] = 0  # an empty image: nothing to do, no device touched
     assert lib.rlic_convolve_noalloc_f32(*args, 3, None) == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_noalloc_call_captured_in_one_graph(gpu_device, dtype):
+    """A whole allocation-free call — kernel copy, pack, trace with its move
+    record and the LDS-staged replays of passes 2..3 — captured in one HIP
+    graph (torch.cuda.graph) and replayed: each replay, with the texture
+    changed in place and eager work and synchronizations in between, gives
+    the bits of an eager call on the same inputs (and of the oracle)."""
+    import torch
+
+    lib = _native.load()
+    noalloc = lib.rlic_convolve_noalloc_f32 if dtype == np.float32 else lib.rlic_convolve_noalloc_f64
+    rng = np.random.default_rng(72)
+    shape, klen, iters = (512, 512), 65, 3  # 2^18 pixels: the replays stage LDS boxes
+    arrays = _fields(rng, shape, klen, dtype)
+    tex, u, v, k = (torch.from_numpy(a).to(gpu_device) for a in arrays)
+    out = torch.empty_like(tex)
+    eager_out = torch.empty_like(tex)
+    bounds = (1, 1, 0, 0)
+
+    def call(dst, stream):
+        a = _args(tex, u, v, k, dst, uv_mode=1, bounds=bounds, iterations=iters)
+        return noalloc(*a, 3, ctypes.c_void_p(stream.cuda_stream))
+
+    side = torch.cuda.Stream(gpu_device)
+    side.wait_stream(torch.cuda.current_stream(gpu_device))
+    with torch.cuda.stream(side):
+        assert call(out, side) == 0, _native.last_error()  # first call: scratch address
+    side.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=side):
+        status = call(out, torch.cuda.current_stream(gpu_device))
+    assert status == 0, _native.last_error()
+    stream = torch.cuda.current_stream(gpu_device)
+    for replay in range(3):
+        new_tex = rng.random(shape).astype(dtype)
+        tex.copy_(torch.from_numpy(new_tex))
+        out.fill_(float("nan"))
+        graph.replay()
+        torch.cuda.synchronize()
+        assert call(eager_out, stream) == 0, _native.last_error()
+        torch.cuda.synchronize()
+        assert torch.equal(out, eager_out), replay
+        if replay == 0:
+            exp = oracle_convolve(new_tex, arrays[1], arrays[2], kernel=arrays[3],
+                                  uv_mode="polarization",
+                                  boundaries={"x": "periodic", "y": "closed"}, iterations=iters)
+            assert np.array_equal(out.cpu().numpy(), exp, equal_nan=True)
