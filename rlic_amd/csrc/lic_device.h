@@ -304,17 +304,34 @@ enum : uint32_t {
   kMoveYMinus = kMoveMoved | kMoveAxisY | kMoveNeg
 };
 
+// A word holds the codes of 8 consecutive steps, the first step in the top
+// nibble: step q in bits 28 - 4q .. 31 - 4q.  (The tracing pass shifts
+// each code bit in at the bottom of the word with one add-with-carry, so
+// the first step's bits end on top.)
+__host__ __device__ inline uint32_t move_code(uint32_t word, int q) {
+  return (word >> (28 - 4 * q)) & 15u;
+}
+
 // Bit 4k set where nibble k of `word` is the stop code.
 __host__ __device__ inline uint32_t stop_nibbles(uint32_t word) {
   return ~(word >> 3) & (word >> 2) & (word >> 1) & word & 0x11111111u;
 }
 
+// Steps of a word before its first stop code (8 when it has none): with
+// the top stop bit at 4k (step 7 - k), (stops << 3) | 1 has its top bit at
+// 4k + 3 (bit 0 without a stop), so (leading zeros + 1) / 4 = 7 - k (8);
+// branch-free, and the argument of clz is never 0.
+__host__ __device__ inline int32_t steps_before_stop(uint32_t word) {
+  const uint32_t stops = stop_nibbles(word);
+  return (__builtin_clz((stops << 3) | 1u) + 1) >> 2;
+}
+
 // Whether a word of 8 recorded steps holds a stop code.  The tracing pass
 // records the stop code again at every step after a line has ended (the
 // line stays on its NaN cell), so a word whose 8 steps were all recorded
-// has one exactly when its last code is the stop code.
+// has one exactly when its last code (the bottom nibble) is the stop code.
 __host__ __device__ inline bool group_has_stop(uint32_t word) {
-  return (word >> 28) == kMoveStop;
+  return (word & 15u) == kMoveStop;
 }
 
 // Positions are in `unit`s: 1 for element indices, the cell size for byte

@@ -135,7 +135,7 @@ __global__ __launch_bounds__(256) void lic_record_stats(const uint4* __restrict_
         for (int wi = 0; wi < 4 && end == nsteps; ++wi) {
           for (int q = 0; q < 8; ++q) {
             const int32_t s = ch * kChunkSteps + 8 * wi + q;
-            if (s < nsteps && ((words[wi] >> (4 * q)) & 15u) == kMoveStop) {
+            if (s < nsteps && move_code(words[wi], q) == kMoveStop) {
               end = s;
               break;
             }
@@ -200,8 +200,9 @@ __global__ __launch_bounds__(256) void lic_replay(const T* __restrict__ xin,
 // lanes of a wave's 8 x 8 pixel square in 64 distinct 4-byte banks
 // (32-bank and 64-bank LDS alike), so an unsheared square reads without
 // bank conflicts.  Moves are decoded two at a time through a 256-entry
-// table indexed by a code byte, holding the two byte-offset deltas as
-// 32-bit integers (one add per step).  The pair table is the block's only
+// table indexed by a code byte (two consecutive steps, the earlier in the
+// high nibble), holding the two byte-offset deltas as 32-bit integers (one
+// add per step).  The pair table is the block's only
 // static LDS, at address 0 (a table address is the code byte times 8); the
 // dynamic LDS after it (a multiple of 64 banks further) holds the box.
 // ---------------------------------------------------------------------------
@@ -304,7 +305,7 @@ __device__ __forceinline__ void walk_group(const Walk& walk, T (&xv)[8],
                                            const int2* pair_delta) {
 #pragma unroll
   for (int h = 0; h < 4; ++h) {
-    const int2 d = pair_delta[(word >> (8 * h)) & 255u];
+    const int2 d = pair_delta[(word >> (24 - 8 * h)) & 255u];
     at += d.x;
     xv[2 * h] = walk(at);
     at += d.y;
@@ -366,7 +367,7 @@ __device__ __forceinline__ void replay_lines(
 #pragma unroll
           for (int e = 0; e < E; ++e)
 #pragma unroll
-            for (int h = 0; h < 4; ++h) d[e][h] = pair_delta[(word[e] >> (8 * h)) & 255u];
+            for (int h = 0; h < 4; ++h) d[e][h] = pair_delta[(word[e] >> (24 - 8 * h)) & 255u];
         }
         T xv[2][8];
 #pragma unroll
@@ -374,7 +375,7 @@ __device__ __forceinline__ void replay_lines(
           if (e < E) {
             if constexpr (!Walk::kTableFirst) {
 #pragma unroll
-              for (int h = 0; h < 4; ++h) d[e][h] = pair_delta[(word[e] >> (8 * h)) & 255u];
+              for (int h = 0; h < 4; ++h) d[e][h] = pair_delta[(word[e] >> (24 - 8 * h)) & 255u];
             }
 #pragma unroll
             for (int h = 0; h < 4; ++h) {
@@ -393,8 +394,7 @@ __device__ __forceinline__ void replay_lines(
       } else {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-          const uint32_t stops = stop_nibbles(word[e]);
-          const int32_t valid = alive[e] ? (stops ? (__builtin_ctz(stops) >> 2) : 8) : 0;
+          const int32_t valid = alive[e] ? steps_before_stop(word[e]) : 0;
           T xv[8];
           walk_group<T>(walk, xv, at[e], word[e], pair_delta);
           // branch-free over the group (see replay_direction)
@@ -441,12 +441,13 @@ __global__ __launch_bounds__(kLdsThreads) void lic_replay_lds(const T* __restric
   const int32_t tid = threadIdx.x;
   const int32_t row_bytes = stride * elem;
   if (tid < 256) {
-    const int32_t lo = lds_move(tid & 15, g, row_bytes, elem);
-    const int32_t hi = lds_move(tid >> 4, g, row_bytes, elem);
+    // a code byte holds two steps, the earlier in its high nibble
+    const int32_t lo = lds_move(tid >> 4, g, row_bytes, elem);
+    const int32_t hi = lds_move(tid & 15, g, row_bytes, elem);
     pair_delta[tid] = make_int2(lo, hi);
     if constexpr (HYBRID) {
-      pair_image[tid] = make_int2(static_cast<int32_t>(move_delta(tid & 15, g, elem)),
-                                  static_cast<int32_t>(move_delta(tid >> 4, g, elem)));
+      pair_image[tid] = make_int2(static_cast<int32_t>(move_delta(tid >> 4, g, elem)),
+                                  static_cast<int32_t>(move_delta(tid & 15, g, elem)));
     }
   }
   const int32_t wave = tid >> 6, lane = tid & 63;

@@ -10,7 +10,8 @@ namespace rlic {
 constexpr int32_t kKernelPad = 8;
 
 // Move-code layout: a direction's steps are cut into chunks of 32 steps
-// (4 words of 8 four-bit codes); chunk c of pixel p (row-major index
+// (4 words of 8 four-bit codes, the first step of a word in its top nibble:
+// move_code); chunk c of pixel p (row-major index
 // i * nx + j) is the 16-byte vector codes4[(c0 + c) * ny * nx + p], so one
 // lane moves 32 steps of codes with one load or store, and the tracing and
 // replay kernels may tile the image differently.
@@ -88,9 +89,13 @@ __device__ __forceinline__ T trace_direction(T acc, T pu, T pv, uint32_t row,
         }
         const T mu = FORWARD ? pu : -pu;  // src/lib.rs:348-351
         const T mv = FORWARD ? pv : -pv;
-        const bool neg_x = __builtin_signbit(mu);
+        // sign tests of mu, mv read pu, pv (negation flips the sign bit), so
+        // that a backward step does not materialize -pu, -pv for them: the
+        // divisions and products take the negation as a source modifier
+        const bool neg_x = FORWARD ? __builtin_signbit(pu) : !__builtin_signbit(pu);
+        const bool neg_y = FORWARD ? __builtin_signbit(pv) : !__builtin_signbit(pv);
         const T tx = time_to_next_pixel<T, FMA, BRANCHLESS>(mu, fx, neg_x);
-        const T ty = time_to_next_pixel<T, FMA, BRANCHLESS>(mv, fy);
+        const T ty = time_to_next_pixel<T, FMA, BRANCHLESS>(mv, fy, neg_y);
         // x edge first when tx < ty; ties and NaN go to y (src/lib.rs:249-269)
         const bool xs = tx < ty;
         // no move when both components are +-0 (src/lib.rs:242-244)
@@ -102,7 +107,7 @@ __device__ __forceinline__ T trace_direction(T acc, T pu, T pv, uint32_t row,
         // time already made.  pos_y is used for y steps, which also happen
         // with mv = -0 (tx and ty both infinite), so it stays mv >= 0.
         const bool pos_x = !neg_x;
-        const bool pos_y = mv >= T(0);
+        const bool pos_y = FORWARD ? pv >= T(0) : pv <= T(0);  // mv >= 0
         const T fy_after_x = macc<FMA>(tx, mv, fy);
         const T fx_after_y = macc<FMA>(ty, mu, fx);
         const T fx_new = xs ? (pos_x ? T(0) : T(1)) : fx_after_y;
@@ -123,10 +128,11 @@ __device__ __forceinline__ T trace_direction(T acc, T pu, T pv, uint32_t row,
           // The code's four bits (moved, edge, y axis, negative direction;
           // the three low bits alone for the stop code) are combined from
           // the step's comparisons as wave lane masks on the scalar unit,
-          // then shifted in one at a time with an add-with-carry (code =
-          // 2 * code + bit).  A line that has ended stays on its NaN cell,
-          // so it records the stop code again at every later step; readers
-          // stop at the first one.
+          // then shifted into the word one at a time with an add-with-carry
+          // (word = 2 * word + bit), so the group's first step ends in the
+          // top nibble (move_code).  A line that has ended stays on its NaN
+          // cell, so it records the stop code again at every later step;
+          // readers stop at the first one.
           const uint64_t m_stop = ~__builtin_amdgcn_ballot_w64(ordered);
           const uint64_t m_nz = __builtin_amdgcn_ballot_w64(mu != T(0)) |
                                 __builtin_amdgcn_ballot_w64(mv != T(0));
@@ -134,14 +140,16 @@ __device__ __forceinline__ T trace_direction(T acc, T pu, T pv, uint32_t row,
           const uint64_t m_edge = __builtin_amdgcn_ballot_w64(edge);
           const uint64_t m_neg = ~((m_x & __builtin_amdgcn_ballot_w64(pos_x)) |
                                    (~m_x & __builtin_amdgcn_ballot_w64(pos_y)));
-          uint32_t code = lane_bit(m_nz & ~m_stop);
-          code = carry_in_bit(code, (m_nz & m_edge) | m_stop);
-          code = carry_in_bit(code, (m_nz & ~m_x) | m_stop);
-          code = carry_in_bit(code, (m_nz & m_neg) | m_stop);
-          word |= code << (4 * q);
+          word = carry_in_bit(word, m_nz & ~m_stop);
+          word = carry_in_bit(word, (m_nz & m_edge) | m_stop);
+          word = carry_in_bit(word, (m_nz & ~m_x) | m_stop);
+          word = carry_in_bit(word, (m_nz & m_neg) | m_stop);
         }
         alive = live;
       }
+      // a last group of fewer than 8 steps: its first step to the top
+      // nibble, "stay" codes after its last
+      if (REC != 0 && rec && nsteps - s0 < 8) word <<= 4 * (8 - (nsteps - s0));
       words[g] = word;
       any_alive = __builtin_amdgcn_ballot_w64(alive) != 0;
     }
@@ -184,7 +192,7 @@ __device__ __forceinline__ T replay_direction(T acc, uint32_t pos,
       uint32_t at[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        pos += delta[(word >> (4 * q)) & 15u];
+        pos += delta[move_code(word, q)];
         at[q] = pos;
       }
       T xv[8];
@@ -199,8 +207,7 @@ __device__ __forceinline__ T replay_direction(T acc, uint32_t pos,
         // no break for the last group: a break lets the compiler sink the
         // gathers past it and wait for each one in turn; steps past the end
         // read "stay" codes (position unchanged, weight from the padding)
-        const uint32_t stops = stop_nibbles(word);
-        const int32_t valid = alive ? (stops ? (__builtin_ctz(stops) >> 2) : 8) : 0;
+        const int32_t valid = alive ? steps_before_stop(word) : 0;
         const int32_t lim = min(valid, nsteps - s0);
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
