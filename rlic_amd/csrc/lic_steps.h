@@ -112,8 +112,14 @@ __device__ __forceinline__ T trace_direction(T acc, T pu, T pv, uint32_t row,
         const T fx_after_y = macc<FMA>(ty, mu, fx);
         const T fx_new = xs ? (pos_x ? T(0) : T(1)) : fx_after_y;
         const T fy_new = xs ? fy_after_x : (pos_y ? T(0) : T(1));
-        fx = moves ? fx_new : fx;
-        fy = moves ? fy_new : fy;
+        // The fractions need no "moves" gate: a lane that does not move
+        // never moves again (an ended line stays on its NaN cell, a zero
+        // vector stays on its cell, whose vector stays zero, polarization
+        // flips included), so its fractions are never read again: they only
+        // feed step times, whose results are masked out of its code bits and
+        // its position update.
+        fx = fx_new;
+        fy = fy_new;
         const uint32_t step = xs ? (pos_x ? unit : 0u - unit) : (pos_y ? row_up : row_down);
         const uint32_t at = moves ? row + step : row;
         const CellVal<T> cv = cell(at);
