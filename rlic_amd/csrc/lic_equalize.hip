@@ -277,7 +277,8 @@ __global__ __launch_bounds__(kSlide * kSlide) void lic_eq_sliding(
   extern __shared__ int32_t box[];
   const int32_t up = (th - 1) / 2, left = (tw - 1) / 2;
   const int32_t i0 = blockIdx.y * kSlide, j0 = blockIdx.x * kSlide;
-  const int32_t bw = kSlide + tw - 1;
+  // the box's sides (LDS launches only: their box fits the LDS)
+  const int32_t bw = LDS ? kSlide + tw - 1 : 0;
   if (LDS) {
     const int32_t bh = kSlide + th - 1;
     for (int32_t k = threadIdx.x; k < bh * bw; k += kSlide * kSlide) {
@@ -309,8 +310,11 @@ __global__ __launch_bounds__(kSlide * kSlide) void lic_eq_sliding(
       }
     }
   } else {
-    const int32_t r0 = max(0, i - up), r1 = min(ny, i - up + th);
-    const int32_t c0 = max(0, j - left), c1 = min(nx, j - left + tw);
+    // (64-bit: i - up + th can pass 2^31 for windows near 2^31 rows)
+    const int32_t r0 = max(0, i - up);
+    const int32_t r1 = static_cast<int32_t>(min<int64_t>(ny, int64_t{i} - up + th));
+    const int32_t c0 = max(0, j - left);
+    const int32_t c1 = static_cast<int32_t>(min<int64_t>(nx, int64_t{j} - left + tw));
     for (int32_t r = r0; r < r1; ++r) {
       const int32_t* row = bins + static_cast<int64_t>(r) * nx;
       for (int32_t c = c0; c < c1; ++c) {
@@ -524,7 +528,8 @@ hipError_t launch_equalize_tiles(int method, const T* img, int64_t sy, int64_t s
                        nbins, st, bins);
     const dim3 grid(static_cast<unsigned>((nx + kSlide - 1) / kSlide),
                     static_cast<unsigned>((ny + kSlide - 1) / kSlide));
-    const int64_t box = int64_t{kSlide + th - 1} * (kSlide + tw - 1) * int64_t{sizeof(int32_t)};
+    const int64_t box = (int64_t{kSlide} + th - 1) * (int64_t{kSlide} + tw - 1) *
+                        static_cast<int64_t>(sizeof(int32_t));
     if (box <= lds_limit) {
       (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&lic_eq_sliding<T, true>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(box));
