@@ -545,16 +545,13 @@ int run_on_device(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
   T* tmp = need_tmp ? reinterpret_cast<T*>(base + off_tmp) : nullptr;
   uint4* code_ptr = replay ? reinterpret_cast<uint4*>(base + off_codes) : nullptr;
   {
-    LaunchTimer timer("lic_pad_kernel", stream);
-    RLIC_HIP_TRY(rlic::launch_pad_kernel<T>(kernel, klen, kpad, stream), "kernel copy");
+    // the cells and the padded weight copy in one launch
+    LaunchTimer timer("lic_pack", stream);
+    RLIC_HIP_TRY(rlic::launch_pack<T>(tex, tsy, tsx, u, usy, usx, v, vsy, vsx, g, c, stream,
+                                      kernel, klen, kpad),
+                 "pack");
   }
   kernel = kpad + pad;
-  {
-    LaunchTimer timer("lic_pack", stream);
-    RLIC_HIP_TRY(
-        rlic::launch_pack<T>(tex, tsy, tsx, u, usy, usx, v, vsy, vsx, g, c, stream),
-        "pack");
-  }
   // trace tiles of 32 x 8 pixels (16 x 4 per wave) in XCD bands of 64
   // tiles: 2.57 vs 2.65 ms per trace launch at 4096^2, L=129 (sweep of
   // bands 4..64, wave widths 8..32 pixels)

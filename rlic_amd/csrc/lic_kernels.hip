@@ -10,6 +10,7 @@
 //               only gathers the new texture, in the same summation order
 //   lic_copy2d  strided 2-D copy (iterations == 0)
 //   lic_repack_texture  texture slot refresh when passes are re-traced
+#include <algorithm>
 #include <type_traits>
 
 #include "lic_device.h"
@@ -23,14 +24,29 @@ namespace rlic {
 // image lands on a guard cell, which holds the values of the cell the
 // boundary rule sends that step to (guard_source) and, in its pad slot, the
 // jump in cells from the guard cell to that cell (0 in image cells).
-// Corner cells are never reached and stay zero.
+// Corner cells are never reached and stay zero.  With `kpad`, the same
+// launch also writes the padded weight copy [pad zeros][kernel][pad zeros]
+// (every thread of the grid takes a share), which saves a small field's
+// call one launch.
 template <typename T>
 __global__ __launch_bounds__(256) void lic_pack(const T* __restrict__ tex, int64_t tsy,
                                                 int64_t tsx, const T* __restrict__ u,
                                                 int64_t usy, int64_t usx,
                                                 const T* __restrict__ v, int64_t vsy,
                                                 int64_t vsx, FieldGeom g,
-                                                Cell<T>* __restrict__ cells) {
+                                                Cell<T>* __restrict__ cells,
+                                                const T* __restrict__ kernel, int32_t klen,
+                                                T* __restrict__ kpad) {
+  if (kpad) {
+    const int64_t threads = static_cast<int64_t>(gridDim.x) * gridDim.y * 256;
+    const int64_t total = int64_t{klen} + 2 * kKernelPad;
+    for (int64_t t = (static_cast<int64_t>(blockIdx.y) * gridDim.x + blockIdx.x) * 256 +
+                     threadIdx.x;
+         t < total; t += threads) {
+      const int64_t k = t - kKernelPad;
+      kpad[t] = (k >= 0 && k < klen) ? kernel[k] : T(0);
+    }
+  }
   const int32_t gj = blockIdx.x * 64 + (threadIdx.x & 63);
   const int32_t gi = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (gi >= g.ny + 2 || gj >= g.nx + 2) return;
@@ -49,17 +65,6 @@ __global__ __launch_bounds__(256) void lic_repack_texture(const T* __restrict__ 
   int32_t i, j;
   if (guard_source(gi, gj, g.ny, g.nx, g.x_left, g.x_right, g.y_left, g.y_right, i, j))
     cells[static_cast<int64_t>(gi) * (g.nx + 2) + gj].x = x[static_cast<int64_t>(i) * g.nx + j];
-}
-
-// dst[0 .. klen + 2*pad) = [pad zeros][kernel][pad zeros]
-template <typename T>
-__global__ __launch_bounds__(256) void lic_pad_kernel(const T* __restrict__ kernel, int32_t klen,
-                                                      int32_t pad, T* __restrict__ dst) {
-  const int32_t total = klen + 2 * pad;
-  for (int32_t t = blockIdx.x * 256 + threadIdx.x; t < total; t += gridDim.x * 256) {
-    const int32_t k = t - pad;
-    dst[t] = (k >= 0 && k < klen) ? kernel[k] : T(0);
-  }
 }
 
 template <typename T>
@@ -619,24 +624,22 @@ int32_t code_chunks(int32_t klen) {
 template <typename T>
 hipError_t launch_pack(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t usy,
                        int64_t usx, const T* v, int64_t vsy, int64_t vsx, const FieldGeom& g,
-                       Cell<T>* cells, hipStream_t stream) {
+                       Cell<T>* cells, hipStream_t stream, const T* kernel, int32_t klen,
+                       T* kpad) {
   dim3 grid((g.nx + 2 + 63) / 64, (g.ny + 2 + 3) / 4);
+  if (kpad) {  // enough threads for the weights of a long kernel on a small image
+    const int64_t need = (int64_t{klen} + 2 * kKernelPad + 255) / 256;
+    const int64_t have = static_cast<int64_t>(grid.x) * grid.y;
+    if (need > have) grid.x = static_cast<unsigned>(std::min<int64_t>(
+                         (need + grid.y - 1) / grid.y, std::max<int64_t>(grid.x, 1024)));
+  }
   hipLaunchKernelGGL(lic_pack<T>, grid, dim3(256), 0, stream, tex, tsy, tsx, u, usy, usx, v, vsy,
-                     vsx, g, cells);
+                     vsx, g, cells, kernel, klen, kpad);
   return hipGetLastError();
 }
 
 int32_t kernel_pad() { return kKernelPad; }
 
-template <typename T>
-hipError_t launch_pad_kernel(const T* kernel, int32_t klen, T* dst, hipStream_t stream) {
-  const int32_t total = klen + 2 * kKernelPad;
-  int32_t blocks = (total + 255) / 256;
-  if (blocks > 1024) blocks = 1024;
-  hipLaunchKernelGGL(lic_pad_kernel<T>, dim3(blocks), dim3(256), 0, stream, kernel, klen,
-                     kKernelPad, dst);
-  return hipGetLastError();
-}
 
 template <typename T>
 hipError_t launch_repack_texture(const T* x, const FieldGeom& g, Cell<T>* cells,
@@ -872,10 +875,9 @@ hipError_t launch_replay_lds_hybrid(const T* x, const T* kernel, int32_t klen, c
 #define RLIC_INSTANTIATE(T)                                                                     \
   template hipError_t launch_pack<T>(const T*, int64_t, int64_t, const T*, int64_t, int64_t,   \
                                      const T*, int64_t, int64_t, const FieldGeom&, Cell<T>*,   \
-                                     hipStream_t);                                             \
+                                     hipStream_t, const T*, int32_t, T*);                      \
   template hipError_t launch_repack_texture<T>(const T*, const FieldGeom&, Cell<T>*,           \
                                                hipStream_t);                                    \
-  template hipError_t launch_pad_kernel<T>(const T*, int32_t, T*, hipStream_t);                \
   template hipError_t launch_copy2d<T>(const T*, int64_t, int64_t, int32_t, int32_t, T*,       \
                                        hipStream_t);                                           \
   template hipError_t launch_trace<T>(const Cell<T>*, const T*, int32_t, const FieldGeom&,     \

@@ -16,19 +16,18 @@ TileMap make_tile_map(int32_t ny, int32_t nx, int32_t band, int32_t wave_w_log2)
 int32_t code_chunks(int32_t klen);
 
 // The field's guarded cell array (guarded_cells(ny, nx) cells) from its
-// strided texture, u and v.
+// strided texture, u and v; with `kpad`, in the same launch, the padded
+// weight copy of `kernel`: kpad (klen + 2 * kernel_pad() elements) =
+// [zeros][kernel][zeros].
 template <typename T>
 hipError_t launch_pack(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t usy,
                        int64_t usx, const T* v, int64_t vsy, int64_t vsx, const FieldGeom& g,
-                       Cell<T>* cells, hipStream_t stream);
+                       Cell<T>* cells, hipStream_t stream, const T* kernel = nullptr,
+                       int32_t klen = 0, T* kpad = nullptr);
 
 // Zeros on each side of the padded kernel-weight copy the streamline
 // kernels read (they load taps in groups of 8 without bounds checks).
 int32_t kernel_pad();
-
-// dst (klen + 2 * kernel_pad() elements) = [zeros][kernel][zeros]
-template <typename T>
-hipError_t launch_pad_kernel(const T* kernel, int32_t klen, T* dst, hipStream_t stream);
 
 template <typename T>
 hipError_t launch_repack_texture(const T* x, const FieldGeom& g, Cell<T>* cells,
@@ -38,7 +37,7 @@ template <typename T>
 hipError_t launch_copy2d(const T* src, int64_t sy, int64_t sx, int32_t ny, int32_t nx, T* dst,
                          hipStream_t stream);
 
-// `kernel` points at the first weight of a padded copy (launch_pad_kernel).
+// `kernel` points at the first weight of a padded copy (launch_pack).
 // codes == nullptr: no recording.  Otherwise codes holds
 // code_chunks(klen) * n_tiles * 256 chunks.
 template <typename T>
