@@ -486,6 +486,17 @@ struct FixedScratch {
   size_t bytes;
 };
 
+// The paired trace (two lanes per pixel, one per direction) where a field
+// has too few pixels to fill the GPU with lic_trace's one lane per pixel:
+// up to RLIC_TRACE_PAIR_MAX pixels (0: never), when it applies.
+constexpr int kTracePairMaxPixels = 1 << 17;
+
+template <typename T>
+bool use_trace_pair(int32_t klen, int32_t ny, int32_t nx) {
+  const int64_t max_px = env_int("RLIC_TRACE_PAIR_MAX", kTracePairMaxPixels);
+  return static_cast<int64_t>(ny) * nx <= max_px && rlic::trace_pair_fits<T>(klen, ny, nx);
+}
+
 // `fixed`: take the call's scratch from that region and allocate nothing
 // (no move record when it does not fit: every pass is traced; no trace
 // statistics, autotuning, graph capture or launch timing, which create HIP
@@ -557,11 +568,17 @@ int run_on_device(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
   // bands 4..64, wave widths 8..32 pixels)
   const rlic::TileMap m = rlic::make_tile_map(ny, nx, env_int("RLIC_TILE_BAND", 64),
                                               env_int("RLIC_WAVE_W_LOG2", 4));
+  // one tracing pass into dst, recording the moves into codes (or not)
+  const bool pair = !stats && use_trace_pair<T>(klen, ny, nx);
+  auto trace = [&](T* dst, uint4* codes) {
+    LaunchTimer timer(pair ? "lic_trace_pair" : "lic_trace", stream);
+    return pair ? rlic::launch_trace_pair<T>(c, kernel, klen, g, m, uv_mode, profile, dst, codes,
+                                             stream)
+                : rlic::launch_trace<T>(c, kernel, klen, g, m, uv_mode, profile, dst, codes,
+                                        stream);
+  };
   if (iterations == 1 && !stats) {
-    LaunchTimer timer("lic_trace", stream);
-    RLIC_HIP_TRY(rlic::launch_trace<T>(c, kernel, klen, g, m, uv_mode, profile, out, nullptr,
-                                       stream),
-                 "trace");
+    RLIC_HIP_TRY(trace(out, nullptr), "trace");
     return RLIC_OK;
   }
   if (replay) {
@@ -569,12 +586,7 @@ int run_on_device(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
     // pass `it` (1-based) writes A when odd, B when even; the last lands in out
     T* a = ((iterations - 1) % 2 == 0) ? out : tmp;
     T* b = (a == out) ? tmp : out;
-    {
-      LaunchTimer timer("lic_trace", stream);
-      RLIC_HIP_TRY(
-          rlic::launch_trace<T>(c, kernel, klen, g, m, uv_mode, profile, a, codes, stream),
-          "trace");
-    }
+    RLIC_HIP_TRY(trace(a, codes), "trace");
     if (stats) {  // measurement: lane work of the pass, from its move record
       DevMem st;
       unsigned long long host_st[2] = {0, 0};
@@ -629,12 +641,7 @@ int run_on_device(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
   }
   for (int64_t it = 1; it <= iterations; ++it) {
     T* dst = (it == iterations) ? out : tmp;
-    {
-      LaunchTimer timer("lic_trace", stream);
-      RLIC_HIP_TRY(rlic::launch_trace<T>(c, kernel, klen, g, m, uv_mode, profile, dst, nullptr,
-                                         stream),
-                   "trace");
-    }
+    RLIC_HIP_TRY(trace(dst, nullptr), "trace");
     if (it != iterations) {
       LaunchTimer timer("lic_repack_texture", stream);
       RLIC_HIP_TRY(rlic::launch_repack_texture<T>(tmp, g, c, stream),

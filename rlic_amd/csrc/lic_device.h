@@ -151,6 +151,16 @@ __device__ __forceinline__ uint32_t lane_copy(uint32_t x) {
   return r;
 }
 
+// x with its sign bit xor-ed with `sign` (0 or 0x80000000; for double it
+// applies to the top word): -x when the bit is set, bit for bit the
+// negation, as one integer instruction.
+__device__ __forceinline__ float xor_sign(float x, uint32_t sign) {
+  return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, x) ^ sign);
+}
+__device__ __forceinline__ double xor_sign(double x, uint32_t sign) {
+  return __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, x) ^ (uint64_t{sign} << 32));
+}
+
 // 2 * x + (this lane's bit of the wave64 lane mask `mask`) in one VALU
 // instruction: the mask is the add's carry-in.
 __device__ __forceinline__ uint32_t carry_in_bit(uint32_t x, uint64_t mask) {

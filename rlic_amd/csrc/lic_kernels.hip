@@ -108,6 +108,71 @@ __global__ __launch_bounds__(256) void lic_trace(const Cell<T>* __restrict__ cel
   out[static_cast<int64_t>(i0) * g.nx + j0] = acc;
 }
 
+// Paired trace for fields too small to fill the GPU: the two directions of
+// a pixel's line run on two lanes of one wave at once (lanes 0..31 forward,
+// 32..63 backward, of the same 32 pixels), so a lane takes half the
+// dependent steps of lic_trace and a field has twice the waves.  The
+// forward lane accumulates as lic_trace does; the backward lane keeps the
+// texture values of its live steps in LDS, which the forward lane then adds
+// in the reference's order (centre, forward steps, backward steps:
+// src/lib.rs:375-383), so the sum is lic_trace's bit for bit.  With REC,
+// each lane records its own direction.  Needs klen odd (both directions
+// kmid steps) and LDS for 32 x (kmid + 1) values (trace_pair_fits).  Blocks
+// of one wave, eight per tile of the TileMap (lic_trace's four waves of 64
+// pixels as eight of 32): one-wave blocks spread a small field's waves over
+// the CUs (measured faster than 512-thread blocks, one per tile, from 32^2
+// to 362^2 pixels).
+template <typename T, bool FMA, bool BRANCHLESS, bool POL, bool REC>
+__global__ __launch_bounds__(64) void lic_trace_pair(
+    const Cell<T>* __restrict__ cells, const T* __restrict__ kernel, int32_t klen, FieldGeom g,
+    TileMap m, T* __restrict__ out, uint4* __restrict__ codes) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int32_t lane = threadIdx.x;
+  const bool fwd = lane < 32;
+  const int32_t k = lane & 31;
+  // a block is one wave: eight blocks take the 256 pixels of a tile
+  int32_t ty, tx, py, px;
+  tile_of_block(static_cast<int32_t>(blockIdx.x) >> 3, m, ty, tx);
+  pixel_of_thread((static_cast<int32_t>(blockIdx.x) & 7) * 32 + k, m, py, px);
+  const int32_t i0 = ty * tile_h(m) + py, j0 = tx * tile_w(m) + px;
+  const bool inside = i0 < g.ny && j0 < g.nx;
+  const int32_t kmid = klen / 2;  // steps of either direction
+  // the backward value of step s of pixel k at vals[32 s] (rows 0..kmid - 1;
+  // row kmid takes the forward lanes' stores)
+  typedef T __attribute__((address_space(3))) lds_t;
+  lds_t* vals = (lds_t*)smem + k;
+  const int64_t p = static_cast<int64_t>(i0) * g.nx + j0;
+  T acc = T(0);
+  int32_t live_steps = 0;  // backward lanes: steps before the line ended
+  if (inside) {
+    const CellGather<T, true> cell(cells, guarded_cells(g.ny, g.nx));
+    const StepGeom sg = make_step_geom(g, cell.unit);
+    const int64_t np = static_cast<int64_t>(g.ny) * g.nx;
+    const uint32_t row0 =
+        static_cast<uint32_t>(i0 + 1) * sg.grow + static_cast<uint32_t>(j0 + 1) * cell.unit;
+    const CellVal<T> c0 = cell(row0);
+    acc = macc<FMA>(kernel[kmid], c0.x, T(0));  // src/lib.rs:375-383
+    // every lane stores (no branch in the step): forward lanes into the
+    // spare row kmid
+    auto keep = [&](int32_t s, T x, bool live) {
+      vals[32 * (fwd ? kmid : s)] = x;
+      live_steps = live ? s + 1 : live_steps;
+    };
+    acc = trace_direction<T, FMA, BRANCHLESS, POL, true, REC, true, true>(
+        acc, c0.u, c0.v, row0, cell, kernel, kmid, kmid, sg, codes, np, p,
+        fwd ? 0 : chunks_of(kmid), false, false, fwd, keep);
+  }
+  __syncthreads();
+  const int32_t nb = __shfl(live_steps, k + 32);  // the backward partner's
+  if (fwd && inside) {
+    for (int32_t s = 0; s < kmid; ++s) {
+      const T next_acc = macc<FMA>(kernel[kmid - 1 - s], vals[32 * s], acc);
+      acc = s < nb ? next_acc : acc;
+    }
+    out[p] = acc;
+  }
+}
+
 // Lane work of a tracing pass, read back from its move record.  The trace
 // kernel runs a wave in groups of 8 steps until every lane's line has ended
 // (or the steps run out); a lane whose line has ended keeps executing the
@@ -706,6 +771,40 @@ hipError_t launch_trace(const Cell<T>* cells, const T* kernel, int32_t klen, con
   }
 }
 
+size_t trace_pair_lds_bytes(int32_t klen, size_t elem) {
+  return size_t{32} * static_cast<size_t>(klen / 2 + 1) * elem;
+}
+
+template <typename T, bool FMA, bool BL, bool POL, bool REC>
+static hipError_t trace_pair_one(const Cell<T>* cells, const T* kernel, int32_t klen,
+                                 const FieldGeom& g, const TileMap& m, T* out, uint4* codes,
+                                 hipStream_t stream) {
+  const size_t lds = trace_pair_lds_bytes(klen, sizeof(T));
+  auto* fn = &lic_trace_pair<T, FMA, BL, POL, REC>;
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+  hipLaunchKernelGGL(fn, dim3(m.n_tiles * 8), dim3(64), lds, stream, cells, kernel, klen, g, m,
+                     out, codes);
+  return hipGetLastError();
+}
+
+template <typename T, bool FMA, bool BL>
+static hipError_t trace_pair_mode(const Cell<T>* cells, const T* kernel, int32_t klen,
+                                  const FieldGeom& g, const TileMap& m, int uv_mode, T* out,
+                                  uint4* codes, hipStream_t stream) {
+  const bool rec = codes != nullptr;
+  if (uv_mode == 1) {
+    return rec ? trace_pair_one<T, FMA, BL, true, true>(cells, kernel, klen, g, m, out, codes,
+                                                         stream)
+               : trace_pair_one<T, FMA, BL, true, false>(cells, kernel, klen, g, m, out, codes,
+                                                          stream);
+  }
+  return rec ? trace_pair_one<T, FMA, BL, false, true>(cells, kernel, klen, g, m, out, codes,
+                                                        stream)
+             : trace_pair_one<T, FMA, BL, false, false>(cells, kernel, klen, g, m, out, codes,
+                                                         stream);
+}
+
 template <typename T>
 hipError_t launch_replay(const T* x, const T* kernel, int32_t klen, const FieldGeom& g,
                          const TileMap& m, int profile, T* out, const uint4* codes,
@@ -759,6 +858,34 @@ int32_t lds_replay_limit_bytes() {
   if (hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
     return 0;
   return limit;
+}
+
+template <typename T>
+bool trace_pair_fits(int32_t klen, int64_t ny, int64_t nx) {
+  const size_t limit = static_cast<size_t>(lds_replay_limit_bytes());
+  return klen % 2 == 1 && trace_pair_lds_bytes(klen, sizeof(T)) <= limit &&
+         fits_buffer(guarded_cells(ny, nx) * static_cast<int64_t>(sizeof(Cell<T>)));
+}
+
+template <typename T>
+hipError_t launch_trace_pair(const Cell<T>* cells, const T* kernel, int32_t klen,
+                             const FieldGeom& g, const TileMap& m, int uv_mode, int profile,
+                             T* out, uint4* codes, hipStream_t stream) {
+  if (!trace_pair_fits<T>(klen, g.ny, g.nx)) return hipErrorInvalidValue;
+  switch (profile & 3) {
+    case 0:
+      return trace_pair_mode<T, false, false>(cells, kernel, klen, g, m, uv_mode, out, codes,
+                                              stream);
+    case 1:
+      return trace_pair_mode<T, true, false>(cells, kernel, klen, g, m, uv_mode, out, codes,
+                                             stream);
+    case 2:
+      return trace_pair_mode<T, false, true>(cells, kernel, klen, g, m, uv_mode, out, codes,
+                                             stream);
+    default:
+      return trace_pair_mode<T, true, true>(cells, kernel, klen, g, m, uv_mode, out, codes,
+                                            stream);
+  }
 }
 
 // LDS replay tile for this kernel length: the largest of 64x64, 32x64 and
@@ -885,6 +1012,9 @@ hipError_t launch_replay_lds_hybrid(const T* x, const T* kernel, int32_t klen, c
   template hipError_t launch_replay<T>(const T*, const T*, int32_t, const FieldGeom&,          \
                                        const TileMap&, int, T*, const uint4*, hipStream_t);    \
   template bool lds_replay_fits<T>(int32_t);                                                   \
+  template bool trace_pair_fits<T>(int32_t, int64_t, int64_t);                               \
+  template hipError_t launch_trace_pair<T>(const Cell<T>*, const T*, int32_t, const FieldGeom&, \
+                                           const TileMap&, int, int, T*, uint4*, hipStream_t);  \
   template LdsTile lds_replay_tile<T>(int32_t);                                                \
   template hipError_t launch_replay_lds<T>(const T*, const T*, int32_t, const FieldGeom&, int, \
                                            T*, const uint4*, hipStream_t);                      \

@@ -45,6 +45,19 @@ hipError_t launch_trace(const Cell<T>* cells, const T* kernel, int32_t klen, con
                         const TileMap& m, int uv_mode, int profile, T* out, uint4* codes,
                         hipStream_t stream);
 
+// Paired trace (both directions of a pixel on two lanes) for fields too
+// small to fill the GPU: trace_pair_fits tells whether it applies (odd
+// klen, LDS for the backward lanes' values, buffer-addressable cells);
+// launch_trace_pair computes what launch_trace does, bit for bit, with the
+// same record, and returns hipErrorInvalidValue where it does not apply.
+size_t trace_pair_lds_bytes(int32_t klen, size_t elem);
+template <typename T>
+bool trace_pair_fits(int32_t klen, int64_t ny, int64_t nx);
+template <typename T>
+hipError_t launch_trace_pair(const Cell<T>* cells, const T* kernel, int32_t klen,
+                             const FieldGeom& g, const TileMap& m, int uv_mode, int profile,
+                             T* out, uint4* codes, hipStream_t stream);
+
 template <typename T>
 hipError_t launch_replay(const T* x, const T* kernel, int32_t klen, const FieldGeom& g,
                          const TileMap& m, int profile, T* out, const uint4* codes,

@@ -41,18 +41,31 @@ __host__ __device__ inline int32_t chunks_of(int32_t nsteps) {
 // side, the opposite edge cell for a periodic one) and, in its pad slot,
 // the jump in cells to that cell (0 in image cells): a step is one add of
 // +-1 cell or +-1 row, the gather, and one shifted add of the jump.
+//
+// LANE_DIR: the direction is the lane's own (forward where fwd_lane), one
+// body for both (the paired trace, lic_trace_pair); FORWARD then names the
+// weights the accumulator takes (the backward lanes' sum is not used).
+// `sink(s, x, live)` sees each step's texture value.
 constexpr int kRuntime = 2;
 
-template <typename T, bool FMA, bool BRANCHLESS, int POL, bool FORWARD, int REC, bool BUF>
+struct NoSink {
+  template <typename T>
+  __device__ __forceinline__ void operator()(int32_t, T, bool) const {}
+};
+
+template <typename T, bool FMA, bool BRANCHLESS, int POL, bool FORWARD, int REC, bool BUF,
+          bool LANE_DIR = false, typename Sink = NoSink>
 __device__ __forceinline__ T trace_direction(T acc, T pu, T pv, uint32_t row,
                                              const CellGather<T, BUF>& cell,
                                              const T* __restrict__ kernel, int32_t kmid,
                                              int32_t nsteps, const StepGeom& sg,
                                              uint4* __restrict__ codes, int64_t np, int64_t p,
                                              int32_t c0, bool pol_on = false,
-                                             bool rec_on = false) {
+                                             bool rec_on = false, bool fwd_lane = true,
+                                             Sink sink = Sink{}) {
   const bool pol = POL == kRuntime ? pol_on : POL != 0;
   const bool rec = REC == kRuntime ? rec_on : REC != 0;
+  const uint32_t lane_sign = fwd_lane ? 0u : 0x80000000u;  // LANE_DIR: negate on backward lanes
   T fx = T(0.5), fy = T(0.5);      // src/lib.rs:315-318
   T last_u = T(0), last_v = T(0);  // src/lib.rs:320-323
   bool alive = true;
@@ -87,13 +100,18 @@ __device__ __forceinline__ T trace_direction(T acc, T pu, T pv, uint32_t row,
           last_u = pu;
           last_v = pv;
         }
-        const T mu = FORWARD ? pu : -pu;  // src/lib.rs:348-351
-        const T mv = FORWARD ? pv : -pv;
+        // src/lib.rs:348-351
+        const T mu = LANE_DIR ? xor_sign(pu, lane_sign) : (FORWARD ? pu : -pu);
+        const T mv = LANE_DIR ? xor_sign(pv, lane_sign) : (FORWARD ? pv : -pv);
         // sign tests of mu, mv read pu, pv (negation flips the sign bit), so
         // that a backward step does not materialize -pu, -pv for them: the
         // divisions and products take the negation as a source modifier
-        const bool neg_x = FORWARD ? __builtin_signbit(pu) : !__builtin_signbit(pu);
-        const bool neg_y = FORWARD ? __builtin_signbit(pv) : !__builtin_signbit(pv);
+        const bool neg_x = LANE_DIR  ? __builtin_signbit(mu)
+                           : FORWARD ? __builtin_signbit(pu)
+                                     : !__builtin_signbit(pu);
+        const bool neg_y = LANE_DIR  ? __builtin_signbit(mv)
+                           : FORWARD ? __builtin_signbit(pv)
+                                     : !__builtin_signbit(pv);
         const T tx = time_to_next_pixel<T, FMA, BRANCHLESS>(mu, fx, neg_x);
         const T ty = time_to_next_pixel<T, FMA, BRANCHLESS>(mv, fy, neg_y);
         // x edge first when tx < ty; ties and NaN go to y (src/lib.rs:249-269)
@@ -107,7 +125,7 @@ __device__ __forceinline__ T trace_direction(T acc, T pu, T pv, uint32_t row,
         // time already made.  pos_y is used for y steps, which also happen
         // with mv = -0 (tx and ty both infinite), so it stays mv >= 0.
         const bool pos_x = !neg_x;
-        const bool pos_y = FORWARD ? pv >= T(0) : pv <= T(0);  // mv >= 0
+        const bool pos_y = LANE_DIR ? mv >= T(0) : FORWARD ? pv >= T(0) : pv <= T(0);  // mv >= 0
         const T fy_after_x = macc<FMA>(tx, mv, fy);
         const T fx_after_y = macc<FMA>(ty, mu, fx);
         const T fx_new = xs ? (pos_x ? T(0) : T(1)) : fx_after_y;
@@ -128,6 +146,7 @@ __device__ __forceinline__ T trace_direction(T acc, T pu, T pv, uint32_t row,
         row = at + jump * unit;
         const T next_acc = macc<FMA>(kw[q], cv.x, acc);  // src/lib.rs:353-360
         acc = live ? next_acc : acc;
+        sink(s0 + q, cv.x, live);
         pu = cv.u;
         pv = cv.v;
         if (REC != 0 && rec) {
