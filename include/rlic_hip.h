@@ -224,6 +224,18 @@ int rlic_convolve_batch_f32(const rlic_field_f32* fields, int64_t n_fields, int 
 int rlic_convolve_batch_f64(const rlic_field_f64* fields, int64_t n_fields, int profile,
                             void* stream);
 
+/* The element checks a batched call's fields must pass first, for all
+ * fields in one launch: the value checks of rlic.convolve (a texture
+ * element below 0, src/rlic/_lib.py:174; a non-finite kernel weight, :194),
+ * which the reference runs per call on the host (`np.any`, `np.isfinite`).
+ * *first_bad = the index of the first failing field, or -1.  Only the
+ * texture and kernel members (and the sizes) of the fields are read.
+ * Returns after the check has completed on `stream`. */
+int rlic_batch_check_values_f32(const rlic_field_f32* fields, int64_t n_fields,
+                                int64_t* first_bad, void* stream);
+int rlic_batch_check_values_f64(const rlic_field_f64* fields, int64_t n_fields,
+                                int64_t* first_bad, void* stream);
+
 /* Serving: fields resident in device memory, advanced one pass per step.
  * No counterpart in the reference (one blocking call per field,
  * src/lib.rs:450-482); every field's passes follow the semantics of that

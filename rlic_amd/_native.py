@@ -88,6 +88,8 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "rlic_server_usage": ([_vp, _i64p, _i64p, _i64p], _int),
     "rlic_convolve_batch_f32": ([ctypes.POINTER(FieldF32), _i64, _int, _vp], _int),
     "rlic_convolve_batch_f64": ([ctypes.POINTER(FieldF64), _i64, _int, _vp], _int),
+    "rlic_batch_check_values_f32": ([ctypes.POINTER(FieldF32), _i64, _i64p, _vp], _int),
+    "rlic_batch_check_values_f64": ([ctypes.POINTER(FieldF64), _i64, _i64p, _vp], _int),
     "rlic_convolve_f32": (_CONVOLVE_ARGTYPES, _int),
     "rlic_convolve_f64": (_CONVOLVE_ARGTYPES, _int),
     # the allocation-free variants take no `location` (device arrays only)

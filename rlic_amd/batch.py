@@ -26,7 +26,7 @@ import numpy as np
 
 from rlic_amd import _core, _native
 from rlic_amd._boundaries import BoundarySet
-from rlic_amd._lib import _KNOWN_UV_MODES, validate
+from rlic_amd._lib import _KNOWN_UV_MODES, _value_flags, validate
 
 _MODE = {"velocity": 0, "polarization": 1}
 _BOUND = {"closed": 0, "periodic": 1}
@@ -106,30 +106,25 @@ def _quick_structure_check(f: Field):
 
 def _first_value_problem(items: list[Field], host: bool) -> int | None:
     """Index of the first field with a negative texture element or a
-    non-finite kernel weight, or None.  Device fields are checked with one
-    reduction per dtype over all fields and a single host synchronisation."""
-    if host:
-        for k, f in enumerate(items):
+    non-finite kernel weight, or None, checking one field at a time: the
+    host batch's check, and a device batch's for the fields ahead of a
+    structurally invalid one.  A valid device batch is checked by the
+    library instead (``rlic_batch_check_values_*``: every field of a dtype
+    in one launch, over the field table the batched call takes)."""
+    for k, f in enumerate(items):
+        if host:
             if np.any(f.texture < 0) or not np.all(np.isfinite(f.kernel)):
                 return k
-        return None
-    if not items:
-        return None
-    import torch
-
-    bad = torch.zeros((), dtype=torch.bool, device=items[0].texture.device)
-    for dt in dict.fromkeys(f.texture.dtype for f in items):
-        texs = [f.texture.reshape(-1) for f in items if f.texture.dtype == dt]
-        bad |= (torch.cat(texs) < 0).any()
-    for dt in dict.fromkeys(f.kernel.dtype for f in items):
-        kerns = [f.kernel.reshape(-1) for f in items if f.kernel.dtype == dt]
-        bad |= (~torch.isfinite(torch.cat(kerns))).any()
-    if not bool(bad):
-        return None
-    for k, f in enumerate(items):  # error path: find the field
-        if bool((f.texture < 0).any()) or bool((~torch.isfinite(f.kernel)).any()):
+        elif any(_value_flags(f.texture, f.kernel)):
             return k
     return None
+
+
+def _raise_for(f: Field):
+    """Raise what ``convolve`` raises for the invalid field f."""
+    validate(f.texture, f.u, f.v, kernel=f.kernel, uv_mode=f.uv_mode,
+             boundaries=f.boundaries, iterations=f.iterations)
+    raise AssertionError("unreachable: the field failed validation above")
 
 
 def convolve_batch(fields: Sequence, *, profile: int | str | None = None, device=None) -> list:
@@ -164,14 +159,13 @@ def convolve_batch(fields: Sequence, *, profile: int | str | None = None, device
         except Exception:
             first_bad = idx
             break
-    value_bad = _first_value_problem(items[:first_bad] if first_bad is not None else items, host)
-    if value_bad is not None:
-        first_bad = value_bad
-    if first_bad is not None:
-        f = items[first_bad]
-        validate(f.texture, f.u, f.v, kernel=f.kernel, uv_mode=f.uv_mode,
-                 boundaries=f.boundaries, iterations=f.iterations)
-        raise AssertionError("unreachable: the field failed validation above")
+    if first_bad is not None:  # an earlier field's element values take precedence
+        value_bad = _first_value_problem(items[:first_bad], host)
+        _raise_for(items[first_bad if value_bad is None else value_bad])
+    if host:
+        value_bad = _first_value_problem(items, host)
+        if value_bad is not None:
+            _raise_for(items[value_bad])
     dev = torch.device(device) if device is not None else (
         torch.device("cuda", torch.cuda.current_device()) if host else items[0].texture.device)
 
@@ -191,9 +185,12 @@ def convolve_batch(fields: Sequence, *, profile: int | str | None = None, device
     lib = _native.load()
     with torch.cuda.device(dev):
         stream = torch.cuda.current_stream(dev).cuda_stream
-        for dtype, struct, entry, tdtype in (
-            (np.dtype("float32"), _native.FieldF32, lib.rlic_convolve_batch_f32, torch.float32),
-            (np.dtype("float64"), _native.FieldF64, lib.rlic_convolve_batch_f64, torch.float64),
+        tables = []  # (field table, struct, batched entry, value check, batch indices)
+        for dtype, struct, entry, check, tdtype in (
+            (np.dtype("float32"), _native.FieldF32, lib.rlic_convolve_batch_f32,
+             lib.rlic_batch_check_values_f32, torch.float32),
+            (np.dtype("float64"), _native.FieldF64, lib.rlic_convolve_batch_f64,
+             lib.rlic_batch_check_values_f64, torch.float64),
         ):
             group = [p for p in prepared if p[1] == dtype]
             if not group:
@@ -209,8 +206,21 @@ def convolve_batch(fields: Sequence, *, profile: int | str | None = None, device
                              ny, nx, k.data_ptr(), k.shape[0], mode,
                              _BOUND[bs.x[0]], _BOUND[bs.x[1]], _BOUND[bs.y[0]], _BOUND[bs.y[1]],
                              iters, out.data_ptr()))
-            arr = np.array(rows, dtype=_native.FIELD_DTYPE)
-            status = entry(arr.ctypes.data_as(ctypes.POINTER(struct)), len(group), prof,
+            tables.append((np.array(rows, dtype=_native.FIELD_DTYPE), struct, entry, check,
+                           [p[0] for p in group]))
+        if not host:  # element values of every field, before any compute
+            value_bad = None
+            for arr, struct, _, check, index in tables:
+                bad = ctypes.c_int64(-1)
+                _native.raise_for_status(check(arr.ctypes.data_as(ctypes.POINTER(struct)),
+                                               len(index), ctypes.byref(bad),
+                                               ctypes.c_void_p(stream)))
+                if bad.value >= 0 and (value_bad is None or index[bad.value] < value_bad):
+                    value_bad = index[bad.value]
+            if value_bad is not None:
+                _raise_for(items[value_bad])
+        for arr, struct, entry, _, index in tables:
+            status = entry(arr.ctypes.data_as(ctypes.POINTER(struct)), len(index), prof,
                            ctypes.c_void_p(stream))
             _native.raise_for_status(status)
     if host:

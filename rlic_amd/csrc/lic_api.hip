@@ -699,6 +699,66 @@ int validate(const T* tex, const T* u, const T* v, int64_t ny, int64_t nx, const
   return RLIC_OK;
 }
 
+// The element checks of rlic.convolve (src/rlic/_lib.py:174, :194) for a
+// whole batch in one launch: *first_bad is the first field whose texture
+// holds an element below 0 or whose kernel a non-finite weight, else -1.
+template <typename T, typename FieldSpec>
+int batch_check_entry(const FieldSpec* fields, int64_t nfields, int64_t* first_bad,
+                      void* stream_ptr) {
+  t_last_error.clear();
+  if (!first_bad) return set_error(RLIC_ERR_INVALID_ARGUMENT, "null first_bad pointer");
+  *first_bad = -1;
+  if (nfields < 0 || (nfields > 0 && !fields))
+    return set_error(RLIC_ERR_INVALID_ARGUMENT, "invalid field list");
+  if (nfields > INT32_MAX) return set_error(RLIC_ERR_INVALID_ARGUMENT, "too many fields");
+  int64_t max_pixels = 0;
+  for (int64_t k = 0; k < nfields; ++k) {
+    const FieldSpec& f = fields[k];
+    if (f.ny < 0 || f.nx < 0 || f.kernel_len < 0)
+      return set_error(RLIC_ERR_INVALID_ARGUMENT, "field %lld: negative size",
+                       static_cast<long long>(k));
+    if (f.ny > 0 && f.nx > 0) {
+      if (const int rc = check_size(f.ny, f.nx, f.kernel_len); rc != RLIC_OK) return rc;
+      if (!f.texture)
+        return set_error(RLIC_ERR_INVALID_ARGUMENT, "field %lld: null texture pointer",
+                         static_cast<long long>(k));
+      max_pixels = std::max(max_pixels, f.ny * f.nx);
+    }
+    if (f.kernel_len > INT32_MAX || (f.kernel_len > 0 && !f.kernel))
+      return set_error(RLIC_ERR_INVALID_ARGUMENT, "field %lld: invalid kernel",
+                       static_cast<long long>(k));
+  }
+  if (nfields == 0) return RLIC_OK;
+  hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+  const size_t table = (sizeof(FieldSpec) * static_cast<size_t>(nfields) + 255) & ~size_t{255};
+  const size_t bytes = table + sizeof(uint32_t) * static_cast<size_t>(nfields);
+  rlic::CallScratch scratch;
+  RLIC_HIP_TRY(scratch.get(bytes, stream), "allocating the check's scratch");
+  char* base = static_cast<char*>(scratch.ptr());
+  auto* d_fields = reinterpret_cast<FieldSpec*>(base);
+  auto* d_flags = reinterpret_cast<uint32_t*>(base + table);
+  RLIC_HIP_TRY(hipMemcpyAsync(d_fields, fields, sizeof(FieldSpec) * static_cast<size_t>(nfields),
+                              hipMemcpyHostToDevice, stream),
+               "uploading the field table");
+  RLIC_HIP_TRY(hipMemsetAsync(d_flags, 0, sizeof(uint32_t) * static_cast<size_t>(nfields), stream),
+               "clearing the check flags");
+  RLIC_HIP_TRY((rlic::launch_check_values<T, FieldSpec>(d_fields, static_cast<int32_t>(nfields),
+                                                        max_pixels, d_flags, stream)),
+               "checking values");
+  std::vector<uint32_t> flags(static_cast<size_t>(nfields));
+  RLIC_HIP_TRY(hipMemcpyAsync(flags.data(), d_flags, sizeof(uint32_t) * flags.size(),
+                              hipMemcpyDeviceToHost, stream),
+               "reading the check flags");
+  RLIC_HIP_TRY(hipStreamSynchronize(stream), "checking values");
+  for (int64_t k = 0; k < nfields; ++k) {
+    if (flags[static_cast<size_t>(k)] != 0) {
+      *first_bad = k;
+      break;
+    }
+  }
+  return RLIC_OK;
+}
+
 template <typename T, typename FieldSpec>
 int batch_entry(const FieldSpec* fields, int64_t nfields, int profile, void* stream_ptr) {
   t_last_error.clear();
@@ -1263,6 +1323,16 @@ int rlic_convolve_batch_f32(const rlic_field_f32* fields, int64_t n_fields, int 
 int rlic_convolve_batch_f64(const rlic_field_f64* fields, int64_t n_fields, int profile,
                             void* stream) {
   return batch_entry<double, rlic_field_f64>(fields, n_fields, profile, stream);
+}
+
+int rlic_batch_check_values_f32(const rlic_field_f32* fields, int64_t n_fields,
+                                int64_t* first_bad, void* stream) {
+  return batch_check_entry<float, rlic_field_f32>(fields, n_fields, first_bad, stream);
+}
+
+int rlic_batch_check_values_f64(const rlic_field_f64* fields, int64_t n_fields,
+                                int64_t* first_bad, void* stream) {
+  return batch_check_entry<double, rlic_field_f64>(fields, n_fields, first_bad, stream);
 }
 
 struct rlic_server {

@@ -72,6 +72,16 @@ int launch_queue(std::vector<BatchField<T>>& fields, const std::vector<const T*>
                  std::vector<Segment>& segs, bool deps, bool pack, int profile,
                  hipStream_t stream, std::string& err);
 
+// The element checks of rlic.convolve (src/rlic/_lib.py:174, :194) for
+// every field of a batch at once, on the device: flags[k] (zeroed by the
+// caller) becomes nonzero when field k's texture holds an element below 0
+// or its kernel a non-finite weight.  `fields`: the field table in device
+// memory; `max_pixels`: the largest texture's element count (sizes the
+// grid).
+template <typename T, typename FieldSpec>
+hipError_t launch_check_values(const FieldSpec* fields, int32_t nfields, int64_t max_pixels,
+                               uint32_t* flags, hipStream_t stream);
+
 // Runs every field of the batch (already validated) in one persistent
 // launch; synchronizes `stream` before returning.
 template <typename T, typename FieldSpec>

@@ -253,6 +253,113 @@ def test_batch_device_validation(gpu_device):
     assert all(o.shape == (4, 4) for o in out)
 
 
+def test_batch_value_check_abi_without_a_gpu():
+    """The batched value check validates its arguments before touching a
+    device: a null result pointer and negative sizes are errors, an empty
+    list passes."""
+    import ctypes
+
+    lib = _native.load()
+    bad = ctypes.c_int64(7)
+    assert lib.rlic_batch_check_values_f32(None, 0, ctypes.byref(bad), None) == 0
+    assert bad.value == -1
+    assert lib.rlic_batch_check_values_f64(None, 0, None, None) < 0
+    assert "first_bad" in _native.last_error()
+    arr = (_native.FieldF32 * 1)()
+    arr[0].ny, arr[0].nx, arr[0].kernel_len = -1, 4, 3
+    assert lib.rlic_batch_check_values_f32(arr, 1, ctypes.byref(bad), None) < 0
+    assert "negative size" in _native.last_error()
+
+
+def _native_first_bad(fields):
+    """The first field failing the library's batched value check, called
+    through the C ABI once per dtype as the batched call does, or None."""
+    import ctypes
+
+    import torch
+
+    lib = _native.load()
+    first = None
+    for struct, check, dt in ((_native.FieldF32, lib.rlic_batch_check_values_f32, torch.float32),
+                              (_native.FieldF64, lib.rlic_batch_check_values_f64, torch.float64)):
+        index = [k for k, f in enumerate(fields) if f.texture.dtype == dt]
+        arr = np.zeros(len(index), dtype=_native.FIELD_DTYPE)
+        for row, k in enumerate(index):
+            t, kern = fields[k].texture, fields[k].kernel
+            arr[row]["texture"], arr[row]["kernel"] = t.data_ptr(), kern.data_ptr()
+            arr[row]["tex_sy"], arr[row]["tex_sx"] = t.stride(0), t.stride(1)
+            arr[row]["ny"], arr[row]["nx"] = t.shape
+            arr[row]["kernel_len"] = kern.shape[0]
+        bad = ctypes.c_int64(-2)
+        stream = torch.cuda.current_stream().cuda_stream
+        assert check(arr.ctypes.data_as(ctypes.POINTER(struct)), len(index), ctypes.byref(bad),
+                     ctypes.c_void_p(stream)) == 0, _native.last_error()
+        assert bad.value >= -1
+        if bad.value >= 0 and (first is None or index[bad.value] < first):
+            first = index[bad.value]
+    return first
+
+
+@pytest.mark.gpu
+def test_batch_value_check_finds_every_position(gpu_device):
+    """The device check of a batch's element values (one launch per dtype)
+    finds a single negative texture element or non-finite kernel weight
+    wherever it sits, in contiguous and strided (transposed) textures of
+    several sizes, reports the first failing field, and passes NaN and
+    signed-zero texture elements, as ``rlic.convolve``'s checks do
+    (src/rlic/_lib.py:174, :194)."""
+    import torch
+
+    rng = np.random.default_rng(8)
+    shapes = [(1, 1), (1, 700), (700, 1), (37, 53), (300, 257), (1024, 1100)]
+    fields = []
+    for ny, nx in shapes:
+        for dt in (torch.float32, torch.float64):
+            tex = torch.rand((ny, nx), dtype=dt, device=gpu_device)
+            tex.view(-1)[:: 97] = float("nan")
+            tex.view(-1)[1:: 89] = -0.0
+            fields.append(Field(tex, tex, tex, torch.rand(17, dtype=dt, device=gpu_device)))
+            trans = torch.rand((nx, ny), dtype=dt, device=gpu_device).t()
+            fields.append(Field(trans, trans, trans, torch.rand(5, dtype=dt, device=gpu_device)))
+    assert _native_first_bad(fields) is None
+    for trial in range(24):
+        k = int(rng.integers(len(fields)))
+        f = fields[k]
+        ny, nx = f.texture.shape
+        i, j = int(rng.integers(ny)), int(rng.integers(nx))
+        if trial % 4 == 3:  # a non-finite kernel weight instead
+            w = int(rng.integers(len(f.kernel)))
+            saved = f.kernel[w].item()
+            f.kernel[w] = float("inf") if trial % 8 == 3 else float("nan")
+            assert _native_first_bad(fields) == k, (k, w)
+            f.kernel[w] = saved
+            continue
+        saved = f.texture[i, j].item()
+        f.texture[i, j] = -1e-30
+        # a later field of the other dtype fails too: the first one is named
+        later = min(len(fields) - 1, k + 1 + int(rng.integers(3)))
+        g = fields[later]
+        g_saved = g.texture[0, 0].item()
+        if later != k:
+            g.texture[0, 0] = -1.0
+        assert _native_first_bad(fields) == k, (k, i, j, later)
+        if later != k:
+            g.texture[0, 0] = g_saved
+        f.texture[i, j] = saved
+    assert _native_first_bad(fields) is None
+    # through the batched call: the first failing field's error, before any compute
+    small = [Field(f.texture, f.u, f.v, f.kernel) for f in fields if f.texture.numel() < 5000]
+    small[-1].texture[0, 0] = -2.0
+    small[3].kernel[1] = float("nan")
+    with pytest.raises(ValueError, match="^Found non-finite value"):
+        convolve_batch(small)
+    small[3].kernel[1] = 0.5
+    with pytest.raises(ValueError, match="^Found invalid texture element"):
+        convolve_batch(small)
+    small[-1].texture[0, 0] = 0.0
+    assert len(convolve_batch(small)) == len(small)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("profile", [0, 1, 2, 3])
 def test_batch_and_serving_follow_each_arithmetic_profile(gpu_device, profile):
