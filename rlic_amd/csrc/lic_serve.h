@@ -31,7 +31,8 @@ struct Resident {
   FieldGeom g{};
   int32_t klen = 0, uv_mode = 0;
   void* block = nullptr;
-  size_t bytes = 0;
+  size_t bytes = 0;        // budget charge (field_bytes)
+  size_t block_bytes = 0;  // allocated: bytes rounded up to a size class
   T* weights = nullptr;
   Cell<T>* cells = nullptr;
   T* buf[2] = {nullptr, nullptr};
@@ -88,6 +89,16 @@ class Server {
   template <typename T>
   int step_dtype(std::vector<int64_t>& finished, std::string& err);
 
+  // Resident blocks come from a cache of released ones (same size class)
+  // before the stream-ordered allocator: with blocks of many sizes coming
+  // and going, hipMallocAsync and hipFreeAsync cost 80 and 220 us a call.
+  // Cached blocks count against the budget as free memory held:
+  // used_ + cached_bytes_ <= budget_ (the cache is trimmed to keep it).
+  static size_t size_class(size_t bytes);
+  hipError_t take_block(size_t charge, size_t cls, void** block);
+  void give_block(void* block, size_t cls);
+  void trim_cache(size_t keep_room);
+
   std::mutex mu_;
   int64_t budget_;
   int64_t used_ = 0;
@@ -96,6 +107,8 @@ class Server {
   int64_t next_handle_ = 1;
   std::map<int64_t, Resident<float>> f32_;
   std::map<int64_t, Resident<double>> f64_;
+  std::multimap<size_t, void*> cache_;  // size class -> released block
+  size_t cached_bytes_ = 0;
 };
 
 }  // namespace rlic

@@ -37,7 +37,51 @@ std::map<int64_t, Resident<double>>& Server::table<double>() { return f64_; }
 Server::~Server() {
   for (auto& kv : f32_) (void)hipFreeAsync(kv.second.block, stream_);
   for (auto& kv : f64_) (void)hipFreeAsync(kv.second.block, stream_);
+  for (auto& kv : cache_) (void)hipFreeAsync(kv.second, stream_);
   (void)hipStreamSynchronize(stream_);
+}
+
+// Eight classes per power of two above 1 MiB (at most 12.5% over the
+// request), 64 KiB steps below.
+size_t Server::size_class(size_t bytes) {
+  size_t step = size_t{64} << 10;
+  if (bytes > (size_t{1} << 20)) {
+    int top = 63 - __builtin_clzll(static_cast<unsigned long long>(bytes));
+    step = size_t{1} << (top - 3);
+  }
+  return (bytes + step - 1) / step * step;
+}
+
+// Frees cached blocks, largest first, until used_ + cached_bytes_ +
+// keep_room fits the budget (or the cache is empty).  Stream-ordered: a
+// freed block's last use was on stream_, or on a side stream the step
+// joined before returning.
+void Server::trim_cache(size_t keep_room) {
+  while (!cache_.empty() &&
+         static_cast<int64_t>(cached_bytes_ + keep_room) + used_ > budget_) {
+    auto last = std::prev(cache_.end());
+    (void)hipFreeAsync(last->second, stream_);
+    cached_bytes_ -= last->first;
+    cache_.erase(last);
+  }
+}
+
+hipError_t Server::take_block(size_t charge, size_t cls, void** block) {
+  auto it = cache_.find(cls);
+  if (it != cache_.end()) {  // reused in stream order after its last use
+    *block = it->second;
+    cached_bytes_ -= cls;
+    cache_.erase(it);
+    return hipSuccess;
+  }
+  trim_cache(charge);
+  return hipMallocAsync(block, cls, stream_);
+}
+
+void Server::give_block(void* block, size_t cls) {
+  cache_.emplace(cls, block);
+  cached_bytes_ += cls;
+  trim_cache(0);
 }
 
 // Replay passes of fields this large run as single-field launches.
@@ -70,7 +114,8 @@ int Server::admit(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
   r.klen = static_cast<int32_t>(klen);
   r.uv_mode = uv_mode;
   r.bytes = bytes;
-  if (hipError_t e = hipMallocAsync(&r.block, bytes, stream_); e != hipSuccess) {
+  r.block_bytes = size_class(bytes);
+  if (hipError_t e = take_block(bytes, r.block_bytes, &r.block); e != hipSuccess) {
     (void)hipGetLastError();
     err = std::string("allocating resident field: ") + hipGetErrorString(e);
     return e == hipErrorOutOfMemory ? RLIC_ERR_OUT_OF_MEMORY : RLIC_ERR_HIP;
@@ -99,7 +144,7 @@ int Server::admit(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
   if (e == hipSuccess && n > 0)
     e = launch_pack<T>(tex, tsy, tsx, u, usy, usx, v, vsy, vsx, r.g, r.cells, stream_);
   if (e != hipSuccess) {
-    (void)hipFreeAsync(r.block, stream_);
+    (void)hipFreeAsync(r.block, stream_);  // not cached: the failed copy may have left it in use
     err = std::string("admitting field: ") + hipGetErrorString(e);
     return RLIC_ERR_HIP;
   }
@@ -284,8 +329,8 @@ int Server::release(int64_t handle, std::string& err) {
   auto go = [&](auto& tab) -> bool {
     auto it = tab.find(handle);
     if (it == tab.end()) return false;
-    (void)hipFreeAsync(it->second.block, stream_);
     used_ -= static_cast<int64_t>(it->second.bytes);
+    give_block(it->second.block, it->second.block_bytes);
     tab.erase(it);
     return true;
   };
