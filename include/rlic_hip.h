@@ -224,17 +224,28 @@ int rlic_convolve_batch_f32(const rlic_field_f32* fields, int64_t n_fields, int 
 int rlic_convolve_batch_f64(const rlic_field_f64* fields, int64_t n_fields, int profile,
                             void* stream);
 
-/* The element checks a batched call's fields must pass first, for all
- * fields in one launch: the value checks of rlic.convolve (a texture
- * element below 0, src/rlic/_lib.py:174; a non-finite kernel weight, :194),
- * which the reference runs per call on the host (`np.any`, `np.isfinite`).
- * *first_bad = the index of the first failing field, or -1.  Only the
- * texture and kernel members (and the sizes) of the fields are read.
- * Returns after the check has completed on `stream`. */
+/* The element checks of rlic.convolve: a texture element below 0
+ * (src/rlic/_lib.py:174) and a non-finite kernel weight (:194), which the
+ * reference runs on the host with NumPy (`np.any(texture < 0)`,
+ * `np.isfinite(kernel)`).  Here one device launch reads the arrays in
+ * place (device memory, strided textures; a NaN texture element passes, as
+ * in the reference) and writes its flags to host-mapped memory; the call
+ * returns after that launch has completed on `stream`.
+ * rlic_check_values_*: one field; *what = bit 0 set when the texture has
+ * an element below 0, bit 1 when the kernel has a non-finite weight.
+ * rlic_batch_check_values_*: every field of a batch (only the texture,
+ * stride, size and kernel members are read); *first_bad = the index of the
+ * first failing field or -1, *what (may be null) = that field's bits. */
+int rlic_check_values_f32(const float* texture, int64_t ny, int64_t nx, int64_t tex_sy,
+                          int64_t tex_sx, const float* kernel, int64_t kernel_len, int* what,
+                          void* stream);
+int rlic_check_values_f64(const double* texture, int64_t ny, int64_t nx, int64_t tex_sy,
+                          int64_t tex_sx, const double* kernel, int64_t kernel_len, int* what,
+                          void* stream);
 int rlic_batch_check_values_f32(const rlic_field_f32* fields, int64_t n_fields,
-                                int64_t* first_bad, void* stream);
+                                int64_t* first_bad, int* what, void* stream);
 int rlic_batch_check_values_f64(const rlic_field_f64* fields, int64_t n_fields,
-                                int64_t* first_bad, void* stream);
+                                int64_t* first_bad, int* what, void* stream);
 
 /* Serving: fields resident in device memory, advanced one pass per step.
  * No counterpart in the reference (one blocking call per field,

@@ -94,12 +94,51 @@ def _any_non_finite(arr: Any) -> bool:
     return bool(np.any(~np.isfinite(arr)))
 
 
+_CHECK_ENTRIES: dict = {}
+
+
+def _device_value_flags(texture: Any, kernel: Any) -> tuple[bool, bool] | None:
+    """The library's one-launch check (``rlic_check_values_*``) for a 2-D
+    float32/float64 texture and a 1-D kernel of its dtype on one ROCm
+    device, or None when the arrays are not of that form."""
+    if not _CHECK_ENTRIES:
+        import torch
+
+        from rlic_amd import _native
+
+        lib = _native.load()
+        _CHECK_ENTRIES.update({torch.float32: lib.rlic_check_values_f32,
+                               torch.float64: lib.rlic_check_values_f64})
+    entry = _CHECK_ENTRIES.get(texture.dtype)
+    if entry is None or kernel.dtype != texture.dtype or texture.dim() != 2 \
+            or kernel.dim() != 1:
+        return None
+    import ctypes
+
+    import torch
+
+    from rlic_amd import _native
+
+    kern = kernel.contiguous()
+    what = ctypes.c_int(0)
+    with torch.cuda.device(texture.device):
+        stream = torch.cuda.current_stream(texture.device).cuda_stream
+        _native.raise_for_status(entry(texture.data_ptr(), texture.shape[0], texture.shape[1],
+                                       texture.stride(0), texture.stride(1), kern.data_ptr(),
+                                       kern.shape[0], ctypes.byref(what), stream))
+    return bool(what.value & 1), bool(what.value & 2)
+
+
 def _value_flags(texture: Any, kernel: Any) -> tuple[bool, bool]:
     """(texture has a negative element, kernel has a non-finite weight);
-    device tensors are reduced on the device and read back together."""
+    float32/float64 device tensors are checked by the library in one
+    launch, other device tensors reduced by torch and read back together."""
     if _is_torch(texture) and _is_torch(kernel) and texture.device == kernel.device \
             and texture.device.type != "cpu" and not texture.is_complex() \
             and not kernel.is_complex():
+        flags = _device_value_flags(texture, kernel)
+        if flags is not None:
+            return flags
         import torch
 
         flags = torch.stack([(texture < 0).any(), (~torch.isfinite(kernel)).any()])

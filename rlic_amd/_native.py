@@ -72,6 +72,7 @@ assert all(FIELD_DTYPE.fields[name][1] == getattr(FieldF32, name).offset
            for name, _ in FieldF32._fields_)
 
 _i64p = ctypes.POINTER(ctypes.c_int64)
+_intp = ctypes.POINTER(ctypes.c_int)
 
 SIGNATURES: dict[str, tuple[list, object]] = {
     "rlic_server_create": ([_i64, _int, _vp], _vp),
@@ -88,8 +89,10 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "rlic_server_usage": ([_vp, _i64p, _i64p, _i64p], _int),
     "rlic_convolve_batch_f32": ([ctypes.POINTER(FieldF32), _i64, _int, _vp], _int),
     "rlic_convolve_batch_f64": ([ctypes.POINTER(FieldF64), _i64, _int, _vp], _int),
-    "rlic_batch_check_values_f32": ([ctypes.POINTER(FieldF32), _i64, _i64p, _vp], _int),
-    "rlic_batch_check_values_f64": ([ctypes.POINTER(FieldF64), _i64, _i64p, _vp], _int),
+    "rlic_batch_check_values_f32": ([ctypes.POINTER(FieldF32), _i64, _i64p, _intp, _vp], _int),
+    "rlic_batch_check_values_f64": ([ctypes.POINTER(FieldF64), _i64, _i64p, _intp, _vp], _int),
+    "rlic_check_values_f32": ([_vp, _i64, _i64, _i64, _i64, _vp, _i64, _intp, _vp], _int),
+    "rlic_check_values_f64": ([_vp, _i64, _i64, _i64, _i64, _vp, _i64, _intp, _vp], _int),
     "rlic_convolve_f32": (_CONVOLVE_ARGTYPES, _int),
     "rlic_convolve_f64": (_CONVOLVE_ARGTYPES, _int),
     # the allocation-free variants take no `location` (device arrays only)

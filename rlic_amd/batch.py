@@ -213,7 +213,7 @@ def convolve_batch(fields: Sequence, *, profile: int | str | None = None, device
             for arr, struct, _, check, index in tables:
                 bad = ctypes.c_int64(-1)
                 _native.raise_for_status(check(arr.ctypes.data_as(ctypes.POINTER(struct)),
-                                               len(index), ctypes.byref(bad),
+                                               len(index), ctypes.byref(bad), None,
                                                ctypes.c_void_p(stream)))
                 if bad.value >= 0 and (value_bad is None or index[bad.value] < value_bad):
                     value_bad = index[bad.value]

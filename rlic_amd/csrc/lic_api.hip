@@ -699,64 +699,142 @@ int validate(const T* tex, const T* u, const T* v, int64_t ny, int64_t nx, const
   return RLIC_OK;
 }
 
-// The element checks of rlic.convolve (src/rlic/_lib.py:174, :194) for a
-// whole batch in one launch: *first_bad is the first field whose texture
-// holds an element below 0 or whose kernel a non-finite weight, else -1.
+// Host-mapped pinned words for the value checks' results, one buffer per
+// host thread, grown on demand and held until the process ends (a call
+// synchronizes before returning, so the next call of the thread may reuse
+// them).  The kernels write the flags there directly: no device-to-host
+// copy, no memset.
+struct PinnedWords {
+  char* host = nullptr;
+  size_t bytes = 0;
+};
+thread_local PinnedWords t_pinned;
+
+hipError_t pinned_words(size_t bytes, char** host, char** dev) {
+  if (t_pinned.bytes < bytes) {
+    size_t want = 4096;
+    while (want < bytes) want *= 2;
+    if (t_pinned.host) (void)hipHostFree(t_pinned.host);
+    t_pinned = PinnedWords{};
+    void* h = nullptr;
+    if (hipError_t e = hipHostMalloc(&h, want, hipHostMallocMapped | hipHostMallocPortable);
+        e != hipSuccess)
+      return e;
+    t_pinned = PinnedWords{static_cast<char*>(h), want};
+  }
+  *host = t_pinned.host;
+  void* d = nullptr;
+  if (hipError_t e = hipHostGetDevicePointer(&d, t_pinned.host, 0); e != hipSuccess) return e;
+  *dev = static_cast<char*>(d);
+  return hipSuccess;
+}
+
+// The element checks of rlic.convolve (src/rlic/_lib.py:174, :194) for
+// `n` fields (sizes already validated) in one launch: *first_bad = the
+// first field whose texture holds an element below 0 or whose kernel a
+// non-finite weight, else -1; *what (if not null) = its problems (bit 0:
+// texture, bit 1: kernel).
+template <typename T>
+int check_values_run(const rlic::CheckField<T>* fields, int64_t n, int64_t* first_bad, int* what,
+                     void* stream_ptr) {
+  *first_bad = -1;
+  if (what) *what = 0;
+  int64_t max_pixels = 0;
+  bool any_values = false;
+  for (int64_t k = 0; k < n; ++k) {
+    const rlic::CheckField<T>& f = fields[k];
+    const int64_t pixels = f.ny > 0 && f.nx > INT64_MAX / f.ny ? INT64_MAX : f.ny * f.nx;
+    max_pixels = std::max(max_pixels, pixels);
+    any_values = any_values || pixels > 0 || f.kernel_len > 0;
+  }
+  if (!any_values) return RLIC_OK;  // nothing to read: no device work
+  hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+  const bool inline_fields = n <= rlic::kCheckInline;
+  const size_t flag_bytes = (sizeof(uint32_t) * 2 * static_cast<size_t>(n) + 255) & ~size_t{255};
+  const size_t table_bytes = inline_fields ? 0 : sizeof(rlic::CheckField<T>) * static_cast<size_t>(n);
+  char* host = nullptr;
+  char* dev = nullptr;
+  RLIC_HIP_TRY(pinned_words(flag_bytes + table_bytes, &host, &dev), "allocating the check flags");
+  auto* flags = reinterpret_cast<uint32_t*>(host);
+  std::memset(flags, 0, sizeof(uint32_t) * 2 * static_cast<size_t>(n));
+  auto* dev_flags = reinterpret_cast<uint32_t*>(dev);
+  rlic::CallScratch scratch;
+  if (inline_fields) {
+    rlic::CheckPack<T> pack{};
+    for (int64_t k = 0; k < n; ++k) pack.f[k] = fields[k];
+    RLIC_HIP_TRY(rlic::launch_check_values<T>(nullptr, &pack, static_cast<int32_t>(n), max_pixels,
+                                              dev_flags, stream),
+                 "checking values");
+  } else {
+    std::memcpy(host + flag_bytes, fields, table_bytes);
+    RLIC_HIP_TRY(scratch.get(table_bytes, stream), "allocating the check's field table");
+    RLIC_HIP_TRY(hipMemcpyAsync(scratch.ptr(), host + flag_bytes, table_bytes,
+                                hipMemcpyHostToDevice, stream),
+                 "uploading the field table");
+    RLIC_HIP_TRY(rlic::launch_check_values<T>(static_cast<const rlic::CheckField<T>*>(scratch.ptr()),
+                                              nullptr, static_cast<int32_t>(n), max_pixels,
+                                              dev_flags, stream),
+                 "checking values");
+  }
+  RLIC_HIP_TRY(hipStreamSynchronize(stream), "checking values");
+  for (int64_t k = 0; k < n; ++k) {
+    const uint32_t bits = (flags[2 * k] ? 1u : 0u) | (flags[2 * k + 1] ? 2u : 0u);
+    if (bits) {
+      *first_bad = k;
+      if (what) *what = static_cast<int>(bits);
+      break;
+    }
+  }
+  return RLIC_OK;
+}
+
+// A batch's fields (only texture, strides, sizes and kernel are read).
 template <typename T, typename FieldSpec>
-int batch_check_entry(const FieldSpec* fields, int64_t nfields, int64_t* first_bad,
+int batch_check_entry(const FieldSpec* fields, int64_t nfields, int64_t* first_bad, int* what,
                       void* stream_ptr) {
   t_last_error.clear();
   if (!first_bad) return set_error(RLIC_ERR_INVALID_ARGUMENT, "null first_bad pointer");
   *first_bad = -1;
+  if (what) *what = 0;
   if (nfields < 0 || (nfields > 0 && !fields))
     return set_error(RLIC_ERR_INVALID_ARGUMENT, "invalid field list");
   if (nfields > INT32_MAX) return set_error(RLIC_ERR_INVALID_ARGUMENT, "too many fields");
-  int64_t max_pixels = 0;
+  std::vector<rlic::CheckField<T>> table(static_cast<size_t>(nfields));
   for (int64_t k = 0; k < nfields; ++k) {
     const FieldSpec& f = fields[k];
     if (f.ny < 0 || f.nx < 0 || f.kernel_len < 0)
       return set_error(RLIC_ERR_INVALID_ARGUMENT, "field %lld: negative size",
                        static_cast<long long>(k));
-    if (f.ny > 0 && f.nx > 0) {
-      if (const int rc = check_size(f.ny, f.nx, f.kernel_len); rc != RLIC_OK) return rc;
-      if (!f.texture)
-        return set_error(RLIC_ERR_INVALID_ARGUMENT, "field %lld: null texture pointer",
-                         static_cast<long long>(k));
-      max_pixels = std::max(max_pixels, f.ny * f.nx);
-    }
-    if (f.kernel_len > INT32_MAX || (f.kernel_len > 0 && !f.kernel))
+    const bool empty = f.ny == 0 || f.nx == 0;  // any size: 64-bit indexing throughout
+    if (!empty && !f.texture)
+      return set_error(RLIC_ERR_INVALID_ARGUMENT, "field %lld: null texture pointer",
+                       static_cast<long long>(k));
+    if (f.kernel_len > 0 && !f.kernel)
       return set_error(RLIC_ERR_INVALID_ARGUMENT, "field %lld: invalid kernel",
                        static_cast<long long>(k));
+    table[static_cast<size_t>(k)] = rlic::CheckField<T>{f.texture, f.tex_sy, f.tex_sx,
+                                                        empty ? 0 : f.ny, empty ? 0 : f.nx,
+                                                        f.kernel, f.kernel_len};
   }
-  if (nfields == 0) return RLIC_OK;
-  hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
-  const size_t table = (sizeof(FieldSpec) * static_cast<size_t>(nfields) + 255) & ~size_t{255};
-  const size_t bytes = table + sizeof(uint32_t) * static_cast<size_t>(nfields);
-  rlic::CallScratch scratch;
-  RLIC_HIP_TRY(scratch.get(bytes, stream), "allocating the check's scratch");
-  char* base = static_cast<char*>(scratch.ptr());
-  auto* d_fields = reinterpret_cast<FieldSpec*>(base);
-  auto* d_flags = reinterpret_cast<uint32_t*>(base + table);
-  RLIC_HIP_TRY(hipMemcpyAsync(d_fields, fields, sizeof(FieldSpec) * static_cast<size_t>(nfields),
-                              hipMemcpyHostToDevice, stream),
-               "uploading the field table");
-  RLIC_HIP_TRY(hipMemsetAsync(d_flags, 0, sizeof(uint32_t) * static_cast<size_t>(nfields), stream),
-               "clearing the check flags");
-  RLIC_HIP_TRY((rlic::launch_check_values<T, FieldSpec>(d_fields, static_cast<int32_t>(nfields),
-                                                        max_pixels, d_flags, stream)),
-               "checking values");
-  std::vector<uint32_t> flags(static_cast<size_t>(nfields));
-  RLIC_HIP_TRY(hipMemcpyAsync(flags.data(), d_flags, sizeof(uint32_t) * flags.size(),
-                              hipMemcpyDeviceToHost, stream),
-               "reading the check flags");
-  RLIC_HIP_TRY(hipStreamSynchronize(stream), "checking values");
-  for (int64_t k = 0; k < nfields; ++k) {
-    if (flags[static_cast<size_t>(k)] != 0) {
-      *first_bad = k;
-      break;
-    }
-  }
-  return RLIC_OK;
+  return check_values_run<T>(table.data(), nfields, first_bad, what, stream_ptr);
+}
+
+// One field's element checks.
+template <typename T>
+int check_entry(const T* texture, int64_t ny, int64_t nx, int64_t sy, int64_t sx, const T* kernel,
+                int64_t kernel_len, int* what, void* stream_ptr) {
+  t_last_error.clear();
+  if (!what) return set_error(RLIC_ERR_INVALID_ARGUMENT, "null result pointer");
+  *what = 0;
+  if (ny < 0 || nx < 0 || kernel_len < 0)
+    return set_error(RLIC_ERR_INVALID_ARGUMENT, "negative size");
+  const bool empty = ny == 0 || nx == 0;  // any size: 64-bit indexing throughout
+  if (!empty && !texture) return set_error(RLIC_ERR_INVALID_ARGUMENT, "null texture pointer");
+  if (kernel_len > 0 && !kernel)
+    return set_error(RLIC_ERR_INVALID_ARGUMENT, "invalid kernel");
+  const rlic::CheckField<T> f{texture, sy, sx, empty ? 0 : ny, empty ? 0 : nx, kernel, kernel_len};
+  int64_t first_bad = -1;
+  return check_values_run<T>(&f, 1, &first_bad, what, stream_ptr);
 }
 
 template <typename T, typename FieldSpec>
@@ -1326,13 +1404,25 @@ int rlic_convolve_batch_f64(const rlic_field_f64* fields, int64_t n_fields, int 
 }
 
 int rlic_batch_check_values_f32(const rlic_field_f32* fields, int64_t n_fields,
-                                int64_t* first_bad, void* stream) {
-  return batch_check_entry<float, rlic_field_f32>(fields, n_fields, first_bad, stream);
+                                int64_t* first_bad, int* what, void* stream) {
+  return batch_check_entry<float, rlic_field_f32>(fields, n_fields, first_bad, what, stream);
 }
 
 int rlic_batch_check_values_f64(const rlic_field_f64* fields, int64_t n_fields,
-                                int64_t* first_bad, void* stream) {
-  return batch_check_entry<double, rlic_field_f64>(fields, n_fields, first_bad, stream);
+                                int64_t* first_bad, int* what, void* stream) {
+  return batch_check_entry<double, rlic_field_f64>(fields, n_fields, first_bad, what, stream);
+}
+
+int rlic_check_values_f32(const float* texture, int64_t ny, int64_t nx, int64_t tex_sy,
+                          int64_t tex_sx, const float* kernel, int64_t kernel_len, int* what,
+                          void* stream) {
+  return check_entry<float>(texture, ny, nx, tex_sy, tex_sx, kernel, kernel_len, what, stream);
+}
+
+int rlic_check_values_f64(const double* texture, int64_t ny, int64_t nx, int64_t tex_sy,
+                          int64_t tex_sx, const double* kernel, int64_t kernel_len, int* what,
+                          void* stream) {
+  return check_entry<double>(texture, ny, nx, tex_sy, tex_sx, kernel, kernel_len, what, stream);
 }
 
 struct rlic_server {

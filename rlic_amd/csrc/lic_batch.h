@@ -73,14 +73,29 @@ int launch_queue(std::vector<BatchField<T>>& fields, const std::vector<const T*>
                  hipStream_t stream, std::string& err);
 
 // The element checks of rlic.convolve (src/rlic/_lib.py:174, :194) for
-// every field of a batch at once, on the device: flags[k] (zeroed by the
-// caller) becomes nonzero when field k's texture holds an element below 0
-// or its kernel a non-finite weight.  `fields`: the field table in device
-// memory; `max_pixels`: the largest texture's element count (sizes the
-// grid).
-template <typename T, typename FieldSpec>
-hipError_t launch_check_values(const FieldSpec* fields, int32_t nfields, int64_t max_pixels,
-                               uint32_t* flags, hipStream_t stream);
+// one field or every field of a batch at once, on the device.
+template <typename T>
+struct CheckField {
+  const T* texture;
+  int64_t tex_sy, tex_sx, ny, nx;
+  const T* kernel;  // contiguous
+  int64_t kernel_len;
+};
+constexpr int32_t kCheckInline = 32;
+template <typename T>
+struct CheckPack {
+  CheckField<T> f[kCheckInline];
+};
+
+// flags[2k] (texture element below 0) and flags[2k + 1] (non-finite
+// weight) of field k become 1; the caller zeroes them first (they may be
+// host-mapped memory).  Fields from `table` (device memory) or, for up to
+// kCheckInline fields, from `pack` (copied into the kernel arguments).
+// `max_pixels`: the largest texture's element count (sizes the grid).
+template <typename T>
+hipError_t launch_check_values(const CheckField<T>* table, const CheckPack<T>* pack,
+                               int32_t nfields, int64_t max_pixels, uint32_t* flags,
+                               hipStream_t stream);
 
 // Runs every field of the batch (already validated) in one persistent
 // launch; synchronizes `stream` before returning.

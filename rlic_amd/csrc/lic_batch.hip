@@ -466,36 +466,57 @@ int launch_queue(std::vector<BatchField<T>>& fields, const std::vector<const T*>
 
 // One field per grid column (blockIdx.x), gridDim.y blocks down the rows
 // of its texture (strided, as given), 256 lanes along each row; block 0
-// also checks the kernel weights.  A NaN texture element passes, as in the
-// reference (`texture < 0` is false for NaN).  HBM-bound: one read of
-// every texture element and kernel weight.
-template <typename T, typename FieldSpec>
-__global__ __launch_bounds__(256) void lic_check_values(const FieldSpec* __restrict__ fields,
-                                                        uint32_t* __restrict__ flags) {
-  const FieldSpec& f = fields[blockIdx.x];
-  bool bad = false;
+// also checks the kernel weights.  flags[2k] = 1: field k has a texture
+// element below 0 (a NaN element passes, as in the reference: `texture <
+// 0` is false for NaN); flags[2k + 1] = 1: a non-finite weight.  Every
+// writer of a word stores the same value.  HBM-bound: one read of every
+// texture element and weight.
+template <typename T>
+__device__ __forceinline__ void check_field(const CheckField<T>& f, uint32_t* flags, uint32_t k) {
+  bool neg = false;
   for (int64_t i = blockIdx.y; i < f.ny; i += gridDim.y) {
     const T* row = f.texture + i * f.tex_sy;
-    for (int64_t j = threadIdx.x; j < f.nx; j += 256) bad = bad || row[j * f.tex_sx] < T(0);
+    for (int64_t j = threadIdx.x; j < f.nx; j += 256) neg = neg || row[j * f.tex_sx] < T(0);
   }
+  if (neg) flags[2 * k] = 1u;
   if (blockIdx.y == 0) {
+    bool bad = false;
     for (int64_t t = threadIdx.x; t < f.kernel_len; t += 256)
       bad = bad || !__builtin_isfinite(f.kernel[t]);
+    if (bad) flags[2 * k + 1] = 1u;
   }
-  if (bad) flags[blockIdx.x] = 1u;  // every writer stores the same value
 }
 
-template <typename T, typename FieldSpec>
-hipError_t launch_check_values(const FieldSpec* fields, int32_t nfields, int64_t max_pixels,
-                               uint32_t* flags, hipStream_t stream) {
+template <typename T>
+__global__ __launch_bounds__(256) void lic_check_values(const CheckField<T>* __restrict__ table,
+                                                        uint32_t* flags) {
+  check_field(table[blockIdx.x], flags, blockIdx.x);
+}
+
+// Up to kCheckInline fields passed in the kernel arguments (no table copy).
+template <typename T>
+__global__ __launch_bounds__(256) void lic_check_values_inline(const CheckPack<T> pack,
+                                                               uint32_t* flags) {
+  check_field(pack.f[blockIdx.x], flags, blockIdx.x);
+}
+
+template <typename T>
+hipError_t launch_check_values(const CheckField<T>* table, const CheckPack<T>* pack,
+                               int32_t nfields, int64_t max_pixels, uint32_t* flags,
+                               hipStream_t stream) {
   if (nfields <= 0) return hipSuccess;
   // about 64 elements per lane of the largest field, at most 4096 blocks
   // per field (blocks of smaller fields past their last row exit at once)
   const int64_t per_block = 256 * 64;
   const auto blocks = static_cast<uint32_t>(
       std::min<int64_t>(4096, std::max<int64_t>(1, (max_pixels + per_block - 1) / per_block)));
-  hipLaunchKernelGGL((lic_check_values<T, FieldSpec>), dim3(static_cast<uint32_t>(nfields), blocks),
-                     dim3(256), 0, stream, fields, flags);
+  const dim3 grid(static_cast<uint32_t>(nfields), blocks);
+  if (pack) {
+    if (nfields > kCheckInline) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((lic_check_values_inline<T>), grid, dim3(256), 0, stream, *pack, flags);
+  } else {
+    hipLaunchKernelGGL((lic_check_values<T>), grid, dim3(256), 0, stream, table, flags);
+  }
   return hipGetLastError();
 }
 
@@ -575,10 +596,11 @@ template int launch_queue<float>(std::vector<BatchField<float>>&, const std::vec
 template int launch_queue<double>(std::vector<BatchField<double>>&,
                                   const std::vector<const double*>&, std::vector<Segment>&, bool,
                                   bool, int, hipStream_t, std::string&);
-template hipError_t launch_check_values<float, rlic_field_f32>(const rlic_field_f32*, int32_t,
-                                                               int64_t, uint32_t*, hipStream_t);
-template hipError_t launch_check_values<double, rlic_field_f64>(const rlic_field_f64*, int32_t,
-                                                                int64_t, uint32_t*, hipStream_t);
+template hipError_t launch_check_values<float>(const CheckField<float>*, const CheckPack<float>*,
+                                               int32_t, int64_t, uint32_t*, hipStream_t);
+template hipError_t launch_check_values<double>(const CheckField<double>*,
+                                                const CheckPack<double>*, int32_t, int64_t,
+                                                uint32_t*, hipStream_t);
 template int batch_run<float, rlic_field_f32>(const rlic_field_f32*, int64_t, int, hipStream_t,
                                               BatchScratch&, std::string&);
 template int batch_run<double, rlic_field_f64>(const rlic_field_f64*, int64_t, int, hipStream_t,

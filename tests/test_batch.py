@@ -260,15 +260,25 @@ def test_batch_value_check_abi_without_a_gpu():
     import ctypes
 
     lib = _native.load()
-    bad = ctypes.c_int64(7)
-    assert lib.rlic_batch_check_values_f32(None, 0, ctypes.byref(bad), None) == 0
-    assert bad.value == -1
-    assert lib.rlic_batch_check_values_f64(None, 0, None, None) < 0
+    bad, what = ctypes.c_int64(7), ctypes.c_int(7)
+    assert lib.rlic_batch_check_values_f32(None, 0, ctypes.byref(bad), ctypes.byref(what),
+                                           None) == 0
+    assert bad.value == -1 and what.value == 0
+    assert lib.rlic_batch_check_values_f64(None, 0, None, None, None) < 0
     assert "first_bad" in _native.last_error()
     arr = (_native.FieldF32 * 1)()
     arr[0].ny, arr[0].nx, arr[0].kernel_len = -1, 4, 3
-    assert lib.rlic_batch_check_values_f32(arr, 1, ctypes.byref(bad), None) < 0
+    assert lib.rlic_batch_check_values_f32(arr, 1, ctypes.byref(bad), None, None) < 0
     assert "negative size" in _native.last_error()
+    arr[0].ny, arr[0].kernel_len = 4, 0  # null texture pointer
+    assert lib.rlic_batch_check_values_f32(arr, 1, ctypes.byref(bad), None, None) < 0
+    assert "null texture" in _native.last_error()
+    assert lib.rlic_check_values_f64(None, 4, 4, 4, 1, None, 0, ctypes.byref(what), None) < 0
+    assert "null texture" in _native.last_error()
+    assert lib.rlic_check_values_f32(None, 0, 4, 4, 1, None, 0, None, None) < 0
+    assert "null result" in _native.last_error()
+    assert lib.rlic_check_values_f32(None, 0, 4, 4, 1, None, 0, ctypes.byref(what), None) == 0
+    assert what.value == 0
 
 
 def _native_first_bad(fields):
@@ -292,8 +302,10 @@ def _native_first_bad(fields):
             arr[row]["kernel_len"] = kern.shape[0]
         bad = ctypes.c_int64(-2)
         stream = torch.cuda.current_stream().cuda_stream
+        what = ctypes.c_int(-1)
         assert check(arr.ctypes.data_as(ctypes.POINTER(struct)), len(index), ctypes.byref(bad),
-                     ctypes.c_void_p(stream)) == 0, _native.last_error()
+                     ctypes.byref(what), ctypes.c_void_p(stream)) == 0, _native.last_error()
+        assert (bad.value == -1) == (what.value == 0) and what.value in (0, 1, 2, 3)
         assert bad.value >= -1
         if bad.value >= 0 and (first is None or index[bad.value] < first):
             first = index[bad.value]
@@ -358,6 +370,52 @@ def test_batch_value_check_finds_every_position(gpu_device):
         convolve_batch(small)
     small[-1].texture[0, 0] = 0.0
     assert len(convolve_batch(small)) == len(small)
+
+
+@pytest.mark.gpu
+def test_single_field_value_check(gpu_device):
+    """``rlic_check_values_*`` (what validation runs for device tensors)
+    agrees with NumPy's checks on every probe: one texture element below 0
+    anywhere in contiguous, transposed and sliced textures, -inf, NaN and
+    -0.0 elements (NaN and -0.0 pass), non-finite weights, empty textures
+    and kernels."""
+    import torch
+
+    from rlic_amd._lib import _device_value_flags, _value_flags
+
+    rng = np.random.default_rng(21)
+    for dt in (torch.float32, torch.float64):
+        base = torch.rand((300, 517), dtype=dt, device=gpu_device)
+        kern = torch.rand(33, dtype=dt, device=gpu_device)
+        views = {"contiguous": base, "transposed": base.t(), "sliced": base[3:250:3, 1::2],
+                 "row": base[7:8], "column": base[:, 9:10]}
+        for name, tex in views.items():
+            assert _device_value_flags(tex, kern) == (False, False), name
+            for _ in range(6):
+                i, j = int(rng.integers(tex.shape[0])), int(rng.integers(tex.shape[1]))
+                for val, neg in ((-1e-30, True), (float("-inf"), True), (float("nan"), False),
+                                 (-0.0, False)):
+                    saved = tex[i, j].item()
+                    tex[i, j] = val
+                    got = _device_value_flags(tex, kern)
+                    want = (bool(np.any(tex.cpu().numpy() < 0)), False)
+                    assert got == want == (neg, False), (name, i, j, val)
+                    tex[i, j] = saved
+        for val in (float("inf"), float("-inf"), float("nan")):
+            w = int(rng.integers(33))
+            k2 = kern.clone()
+            k2[w] = val
+            assert _device_value_flags(base, k2) == (False, True)
+            base[5, 5] = -2.0
+            assert _device_value_flags(base, k2) == (True, True)
+            assert _device_value_flags(base, kern[::2]) == (True, False)  # strided kernel
+            base[5, 5] = 0.5
+        empty = torch.empty((0, 5), dtype=dt, device=gpu_device)
+        assert _device_value_flags(empty, kern) == (False, False)
+        assert _device_value_flags(base, kern[:0]) == (False, False)
+        # other dtypes fall back to torch's reductions
+        assert _device_value_flags(base.half(), kern.half()) is None
+        assert _value_flags(-base.half(), kern.half()) == (True, False)
 
 
 @pytest.mark.gpu
