@@ -27,6 +27,10 @@ hipError_t replay_pass(const T* src, T* dst, const T* kernel, int32_t klen, cons
 // are kept with kernel_pad() zeros on each side (the single-field replay
 // kernels read them so); `weights` points at the first weight.
 template <typename T>
+struct StepWork;
+class SideWork;
+
+template <typename T>
 struct Resident {
   FieldGeom g{};
   int32_t klen = 0, uv_mode = 0;
@@ -87,7 +91,11 @@ class Server {
   template <typename T>
   std::map<int64_t, Resident<T>>& table();
   template <typename T>
-  int step_dtype(std::vector<int64_t>& finished, std::string& err);
+  void plan_dtype(StepWork<T>& w, std::vector<int64_t>& finished);
+  template <typename T>
+  int launch_singles(StepWork<T>& w, const SideWork& side, const int* lane, std::string& err);
+  template <typename T>
+  void commit_dtype(StepWork<T>& w, std::vector<int64_t>& finished);
 
   // Resident blocks come from a cache of released ones (same size class)
   // before the stream-ordered allocator: with blocks of many sizes coming

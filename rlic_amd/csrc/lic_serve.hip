@@ -154,13 +154,19 @@ int Server::admit(const T* tex, int64_t tsy, int64_t tsx, const T* u, int64_t us
   return RLIC_OK;
 }
 
+// One dtype's share of a step: the fields of the persistent launch and the
+// fields whose replay pass runs as a single-field launch.
 template <typename T>
-int Server::step_dtype(std::vector<int64_t>& finished, std::string& err) {
+struct StepWork {
   std::vector<BatchField<T>> fields;
   std::vector<const T*> ksrc;
   std::vector<Segment> segs;
   std::vector<Resident<T>*> stepped, singles;
   std::vector<int64_t> handles, single_handles;
+};
+
+template <typename T>
+void Server::plan_dtype(StepWork<T>& w, std::vector<int64_t>& finished) {
   for (auto& kv : table<T>()) {
     Resident<T>& r = kv.second;
     if (r.done >= r.target) continue;
@@ -171,8 +177,8 @@ int Server::step_dtype(std::vector<int64_t>& finished, std::string& err) {
     }
     if (r.done > 0 && r.recorded &&
         static_cast<int64_t>(r.g.ny) * r.g.nx >= kSingleReplayMinPixels) {
-      singles.push_back(&r);
-      single_handles.push_back(kv.first);
+      w.singles.push_back(&r);
+      w.single_handles.push_back(kv.first);
       continue;
     }
     BatchField<T> f;
@@ -184,7 +190,7 @@ int Server::step_dtype(std::vector<int64_t>& finished, std::string& err) {
     f.uv_mode = r.uv_mode;
     f.iterations = static_cast<int32_t>(std::min<int64_t>(r.target - r.done, INT32_MAX));
     Segment sg{};
-    sg.field = static_cast<int32_t>(fields.size());
+    sg.field = static_cast<int32_t>(w.fields.size());
     sg.prev = -1;
     const int out_buf = r.cur < 0 ? 0 : 1 - r.cur;
     sg.dst = r.buf[out_buf];
@@ -197,35 +203,21 @@ int Server::step_dtype(std::vector<int64_t>& finished, std::string& err) {
       sg.record = 0;
       sg.src = r.buf[r.cur];
     }
-    fields.push_back(f);
-    ksrc.push_back(r.weights);
-    segs.push_back(sg);
-    stepped.push_back(&r);
-    handles.push_back(kv.first);
+    w.fields.push_back(f);
+    w.ksrc.push_back(r.weights);
+    w.segs.push_back(sg);
+    w.stepped.push_back(&r);
+    w.handles.push_back(kv.first);
   }
-  // The single-field replays go to up to RLIC_BATCH_STREAMS (default 4)
-  // side streams, largest first to the least loaded one, where they overlap
-  // with each other and with the persistent launch on the server's stream.
-  SideWork side(stream_);
-  std::vector<int> lane(singles.size(), 0);
-  if (singles.size() > 1) {
-    const char* env = std::getenv("RLIC_BATCH_STREAMS");
-    const int want = std::min<int>(std::min(env && *env ? std::atoi(env) : 4, 16),
-                                   static_cast<int>(singles.size()));
-    int dev = 0;
-    if (want > 1 && hipGetDevice(&dev) == hipSuccess && side.start(dev, want)) {
-      std::vector<int64_t> cost(singles.size());
-      for (size_t k = 0; k < singles.size(); ++k)
-        cost[k] = static_cast<int64_t>(singles[k]->g.ny) * singles[k]->g.nx * singles[k]->klen;
-      lane = SideWork::assign(cost, want);
-    }
-    (void)hipGetLastError();
-  }
-  for (size_t k = 0; k < singles.size(); ++k) {
-    Resident<T>* r = singles[k];
-    const int out_buf = 1 - r->cur;
+}
+
+template <typename T>
+int Server::launch_singles(StepWork<T>& w, const SideWork& side, const int* lane,
+                           std::string& err) {
+  for (size_t k = 0; k < w.singles.size(); ++k) {
+    Resident<T>* r = w.singles[k];
     const hipStream_t s = side.streams.empty() ? stream_ : side.streams[lane[k]];
-    if (hipError_t e = replay_pass<T>(r->buf[r->cur], r->buf[out_buf], r->weights, r->klen,
+    if (hipError_t e = replay_pass<T>(r->buf[r->cur], r->buf[1 - r->cur], r->weights, r->klen,
                                       r->g, profile_, r->codes, s);
         e != hipSuccess) {
       (void)hipGetLastError();
@@ -233,12 +225,65 @@ int Server::step_dtype(std::vector<int64_t>& finished, std::string& err) {
       return RLIC_ERR_HIP;
     }
   }
-  if (!fields.empty()) {
-    const int rc = launch_queue<T>(fields, ksrc, segs, /*deps=*/false, /*pack=*/false, profile_,
-                                   stream_, err);
-    if (rc != RLIC_OK) return rc;
+  return RLIC_OK;
+}
+
+template <typename T>
+void Server::commit_dtype(StepWork<T>& w, std::vector<int64_t>& finished) {
+  for (size_t k = 0; k < w.stepped.size(); ++k) {
+    Resident<T>& r = *w.stepped[k];
+    if (r.done == 0 && r.codes != nullptr) r.recorded = true;
+    r.cur = r.cur < 0 ? 0 : 1 - r.cur;
+    ++r.done;
+    if (r.done == r.target) finished.push_back(w.handles[k]);
   }
-  if (!singles.empty()) {  // the step is synchronous
+  for (size_t k = 0; k < w.singles.size(); ++k) {
+    Resident<T>& r = *w.singles[k];
+    r.cur = 1 - r.cur;
+    ++r.done;
+    if (r.done == r.target) finished.push_back(w.single_handles[k]);
+  }
+}
+
+// Both dtypes' work of a step is enqueued before anything is waited for:
+// the single-field replays of either dtype go to up to RLIC_BATCH_STREAMS
+// (default 4) side streams, largest first to the least loaded one (a
+// float64 pixel-step weighs twice a float32 one), where they overlap with
+// each other and with the two persistent launches on the server's stream.
+int Server::step(std::vector<int64_t>& finished, std::string& err) {
+  std::lock_guard<std::mutex> lock(mu_);
+  StepWork<float> w32;
+  StepWork<double> w64;
+  plan_dtype(w32, finished);
+  plan_dtype(w64, finished);
+  const size_t n_single = w32.singles.size() + w64.singles.size();
+  SideWork side(stream_);
+  std::vector<int> lane(n_single, 0);
+  if (n_single > 1) {
+    const char* env = std::getenv("RLIC_BATCH_STREAMS");
+    const int want = std::min<int>(std::min(env && *env ? std::atoi(env) : 4, 16),
+                                   static_cast<int>(n_single));
+    int dev = 0;
+    if (want > 1 && hipGetDevice(&dev) == hipSuccess && side.start(dev, want)) {
+      std::vector<int64_t> cost;
+      for (Resident<float>* r : w32.singles)
+        cost.push_back(static_cast<int64_t>(r->g.ny) * r->g.nx * r->klen);
+      for (Resident<double>* r : w64.singles)
+        cost.push_back(2 * static_cast<int64_t>(r->g.ny) * r->g.nx * r->klen);
+      lane = SideWork::assign(cost, want);
+    }
+    (void)hipGetLastError();
+  }
+  int rc = launch_singles(w32, side, lane.data(), err);
+  if (rc == RLIC_OK) rc = launch_singles(w64, side, lane.data() + w32.singles.size(), err);
+  if (rc == RLIC_OK && !w32.fields.empty())
+    rc = launch_queue<float>(w32.fields, w32.ksrc, w32.segs, /*deps=*/false, /*pack=*/false,
+                             profile_, stream_, err);
+  if (rc == RLIC_OK && !w64.fields.empty())
+    rc = launch_queue<double>(w64.fields, w64.ksrc, w64.segs, /*deps=*/false, /*pack=*/false,
+                              profile_, stream_, err);
+  if (rc != RLIC_OK) return rc;  // side work still running is waited for by ~SideWork
+  if (n_single > 0) {  // the step is synchronous
     hipError_t e = side.join();
     if (hipError_t se = hipStreamSynchronize(stream_); e == hipSuccess) e = se;
     if (e != hipSuccess) {
@@ -247,27 +292,9 @@ int Server::step_dtype(std::vector<int64_t>& finished, std::string& err) {
       return RLIC_ERR_HIP;
     }
   }
-  for (size_t k = 0; k < stepped.size(); ++k) {
-    Resident<T>& r = *stepped[k];
-    if (r.done == 0 && r.codes != nullptr) r.recorded = true;
-    r.cur = r.cur < 0 ? 0 : 1 - r.cur;
-    ++r.done;
-    if (r.done == r.target) finished.push_back(handles[k]);
-  }
-  for (size_t k = 0; k < singles.size(); ++k) {
-    Resident<T>& r = *singles[k];
-    r.cur = 1 - r.cur;
-    ++r.done;
-    if (r.done == r.target) finished.push_back(single_handles[k]);
-  }
+  commit_dtype(w32, finished);
+  commit_dtype(w64, finished);
   return RLIC_OK;
-}
-
-int Server::step(std::vector<int64_t>& finished, std::string& err) {
-  std::lock_guard<std::mutex> lock(mu_);
-  int rc = step_dtype<float>(finished, err);
-  if (rc == RLIC_OK) rc = step_dtype<double>(finished, err);
-  return rc;
 }
 
 int Server::extend(int64_t handle, int64_t more, std::string& err) {
