@@ -121,11 +121,19 @@ def _device_value_flags(texture: Any, kernel: Any) -> tuple[bool, bool] | None:
 
     kern = kernel.contiguous()
     what = ctypes.c_int(0)
-    with torch.cuda.device(texture.device):
-        stream = torch.cuda.current_stream(texture.device).cuda_stream
+    dev = texture.device
+
+    def run():
+        stream = torch.cuda.current_stream(dev).cuda_stream
         _native.raise_for_status(entry(texture.data_ptr(), texture.shape[0], texture.shape[1],
                                        texture.stride(0), texture.stride(1), kern.data_ptr(),
                                        kern.shape[0], ctypes.byref(what), stream))
+
+    if dev.index is None or dev.index == torch.cuda.current_device():
+        run()
+    else:
+        with torch.cuda.device(dev):
+            run()
     return bool(what.value & 1), bool(what.value & 2)
 
 

@@ -706,6 +706,7 @@ int validate(const T* tex, const T* u, const T* v, int64_t ny, int64_t nx, const
 // copy, no memset.
 struct PinnedWords {
   char* host = nullptr;
+  char* dev = nullptr;  // the same memory's device address (portable: every device)
   size_t bytes = 0;
 };
 thread_local PinnedWords t_pinned;
@@ -720,12 +721,15 @@ hipError_t pinned_words(size_t bytes, char** host, char** dev) {
     if (hipError_t e = hipHostMalloc(&h, want, hipHostMallocMapped | hipHostMallocPortable);
         e != hipSuccess)
       return e;
-    t_pinned = PinnedWords{static_cast<char*>(h), want};
+    void* d = nullptr;
+    if (hipError_t e = hipHostGetDevicePointer(&d, h, 0); e != hipSuccess) {
+      (void)hipHostFree(h);
+      return e;
+    }
+    t_pinned = PinnedWords{static_cast<char*>(h), static_cast<char*>(d), want};
   }
   *host = t_pinned.host;
-  void* d = nullptr;
-  if (hipError_t e = hipHostGetDevicePointer(&d, t_pinned.host, 0); e != hipSuccess) return e;
-  *dev = static_cast<char*>(d);
+  *dev = t_pinned.dev;
   return hipSuccess;
 }
 

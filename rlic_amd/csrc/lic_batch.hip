@@ -473,16 +473,22 @@ int launch_queue(std::vector<BatchField<T>>& fields, const std::vector<const T*>
 // texture element and weight.
 template <typename T>
 __device__ __forceinline__ void check_field(const CheckField<T>& f, uint32_t* flags, uint32_t k) {
+  // no early exit: four independent loads in flight per lane
   bool neg = false;
   for (int64_t i = blockIdx.y; i < f.ny; i += gridDim.y) {
     const T* row = f.texture + i * f.tex_sy;
-    for (int64_t j = threadIdx.x; j < f.nx; j += 256) neg = neg || row[j * f.tex_sx] < T(0);
+    int64_t j = threadIdx.x;
+    for (; j + 768 < f.nx; j += 1024) {
+      const T a = row[j * f.tex_sx], b = row[(j + 256) * f.tex_sx];
+      const T c = row[(j + 512) * f.tex_sx], d = row[(j + 768) * f.tex_sx];
+      neg |= (a < T(0)) | (b < T(0)) | (c < T(0)) | (d < T(0));
+    }
+    for (; j < f.nx; j += 256) neg |= row[j * f.tex_sx] < T(0);
   }
   if (neg) flags[2 * k] = 1u;
   if (blockIdx.y == 0) {
     bool bad = false;
-    for (int64_t t = threadIdx.x; t < f.kernel_len; t += 256)
-      bad = bad || !__builtin_isfinite(f.kernel[t]);
+    for (int64_t t = threadIdx.x; t < f.kernel_len; t += 256) bad |= !__builtin_isfinite(f.kernel[t]);
     if (bad) flags[2 * k + 1] = 1u;
   }
 }
@@ -505,9 +511,9 @@ hipError_t launch_check_values(const CheckField<T>* table, const CheckPack<T>* p
                                int32_t nfields, int64_t max_pixels, uint32_t* flags,
                                hipStream_t stream) {
   if (nfields <= 0) return hipSuccess;
-  // about 64 elements per lane of the largest field, at most 4096 blocks
+  // about 16 elements per lane of the largest field, at most 4096 blocks
   // per field (blocks of smaller fields past their last row exit at once)
-  const int64_t per_block = 256 * 64;
+  const int64_t per_block = 256 * 16;
   const auto blocks = static_cast<uint32_t>(
       std::min<int64_t>(4096, std::max<int64_t>(1, (max_pixels + per_block - 1) / per_block)));
   const dim3 grid(static_cast<uint32_t>(nfields), blocks);
