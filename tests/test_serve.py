@@ -152,6 +152,40 @@ def test_tight_budget_admits_in_turn_and_evicts(gpu_device):
 
 
 @pytest.mark.gpu
+def test_released_blocks_are_reused_with_fresh_contents(gpu_device):
+    """A server hands a released field's device block to the next field of
+    the same size class (and trims its cache to stay within the budget):
+    three rounds of requests over the same few shapes, each round with new
+    textures, kernels and pass counts, every result equal to the oracle's
+    (nothing of an earlier field's cells, weights or move record shows)."""
+    from rlic_amd.serve import Server
+
+    rng = np.random.default_rng(99)
+    shapes = [(40, 56), (56, 40), (33, 33)]
+    lib = _native.load()
+    budget = 3 * int(lib.rlic_server_field_bytes_f64(56, 56, 41, 1, 3))
+    with Server(budget, device=gpu_device) as srv:
+        for rnd in range(3):
+            reqs = []
+            for k in range(6):
+                dtype = np.float64 if k % 2 else np.float32
+                ny, nx = shapes[(k + rnd) % 3]
+                reqs.append(dict(
+                    texture=rng.random((ny, nx)).astype(dtype),
+                    u=(rng.random((ny, nx)) - 0.5).astype(dtype),
+                    v=(rng.random((ny, nx)) - 0.5).astype(dtype),
+                    kernel=rng.random(int(rng.choice([9, 25, 41]))).astype(dtype),
+                    uv_mode=("velocity", "polarization")[(k + rnd) % 2],
+                    boundaries={"x": "periodic", "y": ("closed", "periodic")[k % 2]},
+                    iterations=int(rng.integers(1, 4))))
+            hs = [_submit(srv, r) for r in reqs]
+            srv.run()
+            assert srv.usage()["used_bytes"] == 0 and srv.usage()["resident"] == 0
+            for k, (r, h) in enumerate(zip(reqs, hs)):
+                assert np.array_equal(h.result, _expected(r), equal_nan=True), (rnd, k)
+
+
+@pytest.mark.gpu
 def test_device_tensors_and_edge_cases(gpu_device):
     import torch
 
