@@ -781,8 +781,9 @@ int check_values_run(const rlic::CheckField<T>* fields, int64_t n, int64_t* firs
                  "checking values");
   }
   RLIC_HIP_TRY(hipStreamSynchronize(stream), "checking values");
+  const volatile uint32_t* done_flags = flags;  // written by the device since the memset
   for (int64_t k = 0; k < n; ++k) {
-    const uint32_t bits = (flags[2 * k] ? 1u : 0u) | (flags[2 * k + 1] ? 2u : 0u);
+    const uint32_t bits = (done_flags[2 * k] ? 1u : 0u) | (done_flags[2 * k + 1] ? 2u : 0u);
     if (bits) {
       *first_bad = k;
       if (what) *what = static_cast<int>(bits);
