@@ -511,9 +511,10 @@ hipError_t launch_check_values(const CheckField<T>* table, const CheckPack<T>* p
                                int32_t nfields, int64_t max_pixels, uint32_t* flags,
                                hipStream_t stream) {
   if (nfields <= 0) return hipSuccess;
-  // about 16 elements per lane of the largest field, at most 4096 blocks
-  // per field (blocks of smaller fields past their last row exit at once)
-  const int64_t per_block = 256 * 16;
+  // about 4 elements per lane of the largest field (a small field is
+  // latency-bound: few loads per lane), at most 4096 blocks per field
+  // (blocks of smaller fields past their last row exit at once)
+  const int64_t per_block = 256 * 4;
   const auto blocks = static_cast<uint32_t>(
       std::min<int64_t>(4096, std::max<int64_t>(1, (max_pixels + per_block - 1) / per_block)));
   const dim3 grid(static_cast<uint32_t>(nfields), blocks);
