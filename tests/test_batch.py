@@ -373,6 +373,30 @@ def test_batch_value_check_finds_every_position(gpu_device):
 
 
 @pytest.mark.gpu
+def test_batch_value_check_through_a_table(gpu_device):
+    """More than 32 fields (the table path, not the kernel arguments): a
+    bad field among 70 is found wherever it sits, and so is the first of
+    two."""
+    import torch
+
+    rng = np.random.default_rng(70)
+    fields = []
+    for k in range(70):
+        ny, nx = (int(s) for s in rng.integers(1, 90, size=2))
+        tex = torch.rand((ny, nx), dtype=torch.float32, device=gpu_device)
+        fields.append(Field(tex, tex, tex, torch.rand(9, device=gpu_device)))
+    assert _native_first_bad(fields) is None
+    for k in (0, 33, 37, 69):
+        fields[k].texture[-1, -1] = -1.0
+        assert _native_first_bad(fields) == k
+        fields[50].kernel[3] = float("nan")
+        assert _native_first_bad(fields) == min(k, 50)
+        fields[50].kernel[3] = 0.5
+        fields[k].texture[-1, -1] = 0.5
+    assert _native_first_bad(fields) is None
+
+
+@pytest.mark.gpu
 def test_single_field_value_check(gpu_device):
     """``rlic_check_values_*`` (what validation runs for device tensors)
     agrees with NumPy's checks on every probe: one texture element below 0
